@@ -1,0 +1,180 @@
+/*
+ * rigidfibers.h -- C ABI of the MI355X-native rigid-fiber slender-body
+ * timestep (librigidfibers.so).
+ *
+ * Drop-in for the simulation surface of ericwolter/rigidfibers_kth:
+ *   - the line-ordered run-parameter file  (src/parameters.cc:124-180,
+ *     fortran/advect_fibers.f90:182-222),
+ *   - the XcT_init layout file             (src/parameters.cc:186-227,
+ *     fortran/advect_fibers.f90:192-196,316-322),
+ *   - the per-step text outputs            (fortran/advect_fibers.f90:529-539,
+ *     fortran/coeffs_and_vels.f90:55-72,633-642),
+ *   - the Simulation object and its step   (src/simulation.h:36-87,
+ *     src/simulation.cu:279-301; the Fortran loop body
+ *     fortran/advect_fibers.f90:437-593).
+ *
+ * Plain C types only: no HIP, torch or C++ types cross this boundary.
+ * All arrays are FP64 and caller-owned unless stated otherwise.
+ *   positions / orientations / velocities : 3N doubles, fiber-major xyz
+ *   coefficient vector x and rhs b        : n = 3*M*N doubles, index
+ *                                           3*M*i + 3*(k-1) + c
+ *   matrix A                               : n x n column-major
+ *
+ * Return codes: RF_OK (0) or a negative RF_E* code; rf_last_error() gives a
+ * message for the calling thread.  One rf_sim handle per thread; each handle
+ * owns its device memory and one HIP stream on the current device.
+ */
+#ifndef RIGIDFIBERS_H_
+#define RIGIDFIBERS_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RF_OK 0
+#define RF_EINVAL (-1)  /* bad argument / parameter */
+#define RF_EIO (-2)     /* file could not be opened or parsed */
+#define RF_ENOMEM (-3)  /* host or device allocation failed */
+#define RF_EHIP (-4)    /* HIP runtime error (incl. no GPU / kernel image missing) */
+#define RF_ENOCONV (-5) /* solver failed, including the LU fallback */
+
+/* precision modes */
+#define RF_PREC_FP64 0  /* FP64 assembly, FP64 matrix, FP64 solve (reference) */
+#define RF_PREC_MIXED 1 /* FP32 matrix (FP64 near field), FP64 residual/Arnoldi */
+
+/* initial guess for GMRES */
+#define RF_WARM_PREV 0    /* previous step's coefficients (fortran/advect_fibers.f90:464) */
+#define RF_WARM_ZERO 1    /* x0 = 0 every step */
+#define RF_WARM_EXTRAP 2  /* 2 x_n - x_{n-1} (linear extrapolation in time) */
+#define RF_WARM_LEARNED 3 /* x_{n} + MLP(now) - MLP(prev): learned increment */
+
+/* output number formats */
+#define RF_FMT_F16_8 0  /* '(*(F16.8))' as the Fortran's *.out / tests/reference *.ref */
+#define RF_FMT_F24_16 1 /* '(2F24.16)' as XcTData / VelRotData / Coeffvec */
+
+/* solver recorded in rf_stats.last_solver */
+#define RF_SOLVER_GMRES 0
+#define RF_SOLVER_LU 1
+#define RF_SOLVER_LU_FALLBACK 2
+
+/* Run parameters.  Lines 0..13 of the v1 parameter file, in order, then the
+ * extensions this build adds (all default to the reference behaviour). */
+typedef struct rf_params {
+    int32_t run_label;                    /* line 0 */
+    int32_t layout_label;                 /* line 1 */
+    int32_t num_terms;                    /* line 2: M, 1..8 */
+    double slenderness;                   /* line 3: eps */
+    double timestep;                      /* line 4: dt */
+    int32_t num_timesteps;                /* line 5 */
+    int32_t save_interval;                /* line 6 */
+    int32_t saves_per_file;               /* line 7 */
+    int32_t num_quad_intervals;           /* line 8: NoQI */
+    int32_t num_quad_points_per_interval; /* fixed 3 (src/parameters.cc:36) */
+    int32_t use_analytical_integration;   /* line 9 */
+    int32_t use_direct_solver;            /* line 10: 1 LU, 0 GMRES */
+    int32_t gmres_restart;                /* line 11 */
+    int32_t gmres_max_iters;              /* line 12 */
+    double gmres_tol;                     /* line 13 */
+    double external_force[3];             /* per fiber; (0,0,-1) (advect_fibers.f90:307-312) */
+    /* extensions */
+    int32_t precision;  /* RF_PREC_* */
+    int32_t warm_start; /* RF_WARM_* */
+    int32_t profile;    /* 1: time every GEMV / assembly launch with HIP events */
+    int32_t lu_block;   /* LU panel width (0 = default) */
+    double mixed_inner_tol; /* mixed mode: tolerance of the FP32-operator solve (0 = default 1e-10) */
+    int32_t reserved[8];
+} rf_params;
+
+typedef struct rf_stats {
+    int64_t steps_taken;
+    int32_t last_gmres_iters;      /* iterations of the last solve (0 for LU) */
+    int32_t last_solver;           /* RF_SOLVER_* */
+    double last_backward_error;    /* ||b - A x|| / ||b|| of the last solve */
+    int64_t total_gmres_iters;
+    /* phase times of the last step and totals, milliseconds (HIP events) */
+    double ms_assemble, ms_solve, ms_velocities, ms_advance;
+    double total_ms_assemble, total_ms_solve, total_ms_velocities, total_ms_advance;
+    /* per-kernel timing, filled when rf_params.profile != 0 */
+    int64_t gemv_launches;          /* system-matrix GEMV launches (GMRES + residuals) */
+    double gemv_ms_total;
+    int64_t assemble_launches;      /* assembly kernel launches */
+    double assemble_ms_total;
+    int64_t velocity_gemv_launches;
+    double velocity_gemv_ms_total;
+    int64_t lu_factorizations;
+    double lu_ms_total;
+    double matrix_bytes;            /* bytes of one system-matrix copy (n*lda*elem) */
+    int32_t matrix_elem_bytes;      /* 8 FP64, 4 FP32 */
+    int32_t lda;
+    double reserved[8];
+} rf_stats;
+
+typedef struct rf_sim rf_sim;
+
+/* Defaults: M=5, eps=0.01, dt=0.1, NoQI=8, analytic, GMRES(10,100,1e-5),
+ * force (0,0,-1), FP64, previous-step warm start. */
+void rf_params_default(rf_params *p);
+
+/* v1 parameter file (src/parameters.cc:124-180; fortran/advect_fibers.f90:182-222).
+ * Lines after the last mandatory one may hold "key value" extensions:
+ * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1. */
+int rf_parse_parameters(const char *path, rf_params *out);
+
+/* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per
+ * fiber, any whitespace.  *pos and *orient are library-allocated (3N doubles
+ * each); release with rf_free. */
+int rf_parse_layout(const char *path, int32_t *n, double **pos, double **orient);
+void rf_free(void *p);
+
+/* Simulation(Configuration) (src/simulation.cu:40-58).  Copies the inputs,
+ * normalises the orientations (fortran/advect_fibers.f90:332-340) and
+ * allocates all device memory. */
+int rf_create(const rf_params *params, int32_t num_fibers, const double *pos, const double *orient,
+              rf_sim **out);
+void rf_destroy(rf_sim *sim);
+
+/* One timestep: assemble, solve, velocities, advance (Euler on the first
+ * call, second-order multistep afterwards).  Simulation::step
+ * (src/simulation.cu:279-301); fortran/advect_fibers.f90:437-523. */
+int rf_step(rf_sim *sim);
+
+/* The four phases of rf_step, for tests and timing. */
+int rf_assemble(rf_sim *sim);            /* assemble_matrix[_an] + assemble_rhs[_an] */
+int rf_solve(rf_sim *sim);               /* dgesv / dGMRES (coeffs_and_vels.f90:10-532) */
+int rf_update_velocities(rf_sim *sim);   /* compute_velocities[_an] (coeffs_and_vels.f90:543-746) */
+int rf_advance(rf_sim *sim);             /* update_fibers[_firststep] (advect_fibers.f90:498-523) */
+
+/* Copies out of the current state (positions of the newest time level). */
+int rf_get_state(const rf_sim *sim, double *pos, double *orient);
+int rf_get_velocities(const rf_sim *sim, double *vel, double *rot);
+int rf_get_coefficients(const rf_sim *sim, double *x);
+int rf_get_rhs(const rf_sim *sim, double *b);
+/* The assembled system matrix, promoted to FP64, into a caller buffer with
+ * leading dimension lda >= n (test hook; the reference's AMat.out). */
+int rf_get_matrix(const rf_sim *sim, double *A, int64_t lda);
+int rf_get_stats(const rf_sim *sim, rf_stats *stats);
+/* Sets the coefficient vector (used as the next GMRES initial guess). */
+int rf_set_coefficients(rf_sim *sim, const double *x);
+
+/* y = A x with the assembled device matrix (FP64 arithmetic), n doubles. */
+int rf_matvec(rf_sim *sim, const double *x, double *y);
+
+/* POS.out ORIENT.out TRANSVel.out ROTVel.out XVec.out BVec.out of the last
+ * step into dir, RF_FMT_F16_8 or RF_FMT_F24_16. */
+int rf_write_step_outputs(const rf_sim *sim, const char *dir, int32_t format);
+
+/* Number of fibers / unknowns of a handle. */
+int32_t rf_num_fibers(const rf_sim *sim);
+int64_t rf_num_unknowns(const rf_sim *sim);
+
+/* Message of the last failing call on this thread. */
+const char *rf_last_error(void);
+/* Library version string. */
+const char *rf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RIGIDFIBERS_H_ */

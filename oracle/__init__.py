@@ -52,6 +52,11 @@ def lib():
                                    ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_compute_velocities.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                              _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_assemble_rhs_rows.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                            _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, _dp]
+        L.orc_compute_velocities_rows.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                  _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                                  ctypes.c_int, _ip, _dp, _dp]
         L.orc_lu_solve.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_lu_solve.restype = ctypes.c_int
         L.orc_gmres.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
@@ -164,6 +169,38 @@ def compute_velocities(analytic, eps, pos, orient, force, M, noqi, coeff):
     lib().orc_compute_velocities(int(analytic), eps, N, M, _p(pos), _p(orient), _p(force), pv.size, _p(pv),
                                  _p(wv), _p(lv), _p(coeff), _p(vel), _p(rot))
     return vel, rot
+
+
+def assemble_rhs_rows(analytic, eps, pos, orient, force, M, noqi, targets):
+    pos, orient, force = _f64(pos).ravel(), _f64(orient).ravel(), _f64(force).ravel()
+    N = pos.size // 3
+    pv, wv, lv = _setup(M, noqi)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    b = np.empty(3 * M * t.size)
+    lib().orc_assemble_rhs_rows(int(analytic), eps, N, M, _p(pos), _p(orient), _p(force), pv.size, _p(pv),
+                                _p(wv), _p(lv), t.size, t.ctypes.data_as(_ip), _p(b))
+    return b
+
+
+def compute_velocities_rows(analytic, eps, pos, orient, force, M, noqi, coeff, targets):
+    pos, orient, force, coeff = (_f64(a).ravel() for a in (pos, orient, force, coeff))
+    N = pos.size // 3
+    pv, wv, lv = _setup(M, noqi)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    vel = np.empty(3 * t.size)
+    rot = np.empty(3 * t.size)
+    lib().orc_compute_velocities_rows(int(analytic), eps, N, M, _p(pos), _p(orient), _p(force), pv.size, _p(pv),
+                                      _p(wv), _p(lv), _p(coeff), t.size, t.ctypes.data_as(_ip), _p(vel), _p(rot))
+    return vel, rot
+
+
+def matrix_block(analytic, eps, xi, ti, xj, tj, M, noqi):
+    """Block (i, j) of A: the off-diagonal block of the two-fiber system."""
+    pos = np.concatenate([_f64(xi), _f64(xj)])
+    ori = np.concatenate([_f64(ti), _f64(tj)])
+    A = assemble_matrix(analytic, eps, pos, ori, M, noqi)
+    m = 3 * M
+    return A[:m, m:]
 
 
 def lu_solve(A, b):

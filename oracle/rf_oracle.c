@@ -403,19 +403,27 @@ void orc_assemble_rhs(int analytic, double eps, int N, int M,
                       int LQ, const double *pv, const double *wv,
                       const double *lvec, double *b)
 {
+    orc_assemble_rhs_rows(analytic, eps, N, M, pos, orient, force, LQ, pv, wv, lvec, N, NULL, b);
+}
+
+void orc_assemble_rhs_rows(int analytic, double eps, int N, int M,
+                           const double *pos, const double *orient, const double *force,
+                           int LQ, const double *pv, const double *wv,
+                           const double *lvec, int ntargets, const int *targets, double *b)
+{
     consts_t K;
     make_consts(eps, M, &K);
-    int n = 3 * M * N;
-    memset(b, 0, sizeof(double) * n);
+    memset(b, 0, sizeof(double) * 3 * M * (size_t)ntargets);
 #pragma omp parallel
     {
         double *Gmat = (double *)malloc(sizeof(double) * LQ * 3);
         double *fx = (double *)malloc(sizeof(double) * LQ * 3);
         double *fy = fx + LQ, *fz = fx + 2 * LQ;
 #pragma omp for schedule(dynamic, 1)
-        for (int filno = 0; filno < N; ++filno) {
+        for (int ti = 0; ti < ntargets; ++ti) {
+            const int filno = targets ? targets[ti] : ti;
             const double *xc = pos + 3 * filno, *ta = orient + 3 * filno;
-            double *brow = b + 3 * M * filno;
+            double *brow = b + 3 * M * (size_t)ti;
             for (int fno = 0; fno < N; ++fno) {
                 if (fno == filno) continue;
                 const double *xcb = pos + 3 * fno, *tb = orient + 3 * fno;
@@ -546,6 +554,16 @@ void orc_compute_velocities(int analytic, double eps, int N, int M,
                             int LQ, const double *pv, const double *wv, const double *lvec,
                             const double *coeff, double *vel, double *rot)
 {
+    orc_compute_velocities_rows(analytic, eps, N, M, pos, orient, force, LQ, pv, wv, lvec, coeff,
+                                N, NULL, vel, rot);
+}
+
+void orc_compute_velocities_rows(int analytic, double eps, int N, int M,
+                                 const double *pos, const double *orient, const double *force,
+                                 int LQ, const double *pv, const double *wv, const double *lvec,
+                                 const double *coeff, int ntargets, const int *targets,
+                                 double *vel, double *rot)
+{
     double c = log(eps * eps * exp(1.0));
     double d = -c, e = 2.0, mubar = d;
     double *Fall = (double *)malloc(sizeof(double) * 3 * (size_t)N * LQ);
@@ -556,7 +574,8 @@ void orc_compute_velocities(int analytic, double eps, int N, int M,
         double *G0 = (double *)malloc(sizeof(double) * LQ * 3);
         double *Gall = (double *)malloc(sizeof(double) * 6 * (M + 1));
 #pragma omp for schedule(dynamic, 1)
-        for (int filno = 0; filno < N; ++filno) {
+        for (int ti = 0; ti < ntargets; ++ti) {
+            const int filno = targets ? targets[ti] : ti;
             const double *xc = pos + 3 * filno, *ta = orient + 3 * filno;
             const double *Fa = force + 3 * filno;
             double tdF = ta[0] * Fa[0] + ta[1] * Fa[1] + ta[2] * Fa[2];
@@ -610,8 +629,8 @@ void orc_compute_velocities(int analytic, double eps, int N, int M,
                 }
             }
             for (int a = 0; a < 3; ++a) {
-                vel[3 * filno + a] = V[a] / mubar;
-                rot[3 * filno + a] = R[a] / mubar;
+                vel[3 * ti + a] = V[a] / mubar;
+                rot[3 * ti + a] = R[a] / mubar;
             }
         }
         free(G0);

@@ -53,6 +53,11 @@ void orc_assemble_rhs(int analytic, double eps, int N, int M,
                       const double *pos, const double *orient, const double *force,
                       int LQ, const double *pv, const double *wv,
                       const double *lvec, double *b);
+/* Rows of b for the fibers targets[0..ntargets) only (b: 3M*ntargets). */
+void orc_assemble_rhs_rows(int analytic, double eps, int N, int M,
+                           const double *pos, const double *orient, const double *force,
+                           int LQ, const double *pv, const double *wv,
+                           const double *lvec, int ntargets, const int *targets, double *b);
 /* Matrix-free y = A x (numeric kernel): fortran/matrix_assembly.f90:159-265. */
 void orc_mat_prod(double eps, int N, int M, const double *pos, const double *orient,
                   int LQ, const double *pv, const double *wv, const double *lvec,
@@ -63,6 +68,13 @@ void orc_compute_velocities(int analytic, double eps, int N, int M,
                             const double *pos, const double *orient, const double *force,
                             int LQ, const double *pv, const double *wv, const double *lvec,
                             const double *coeff, double *vel, double *rot);
+
+/* V, Omega of the fibers targets[0..ntargets) only (vel, rot: 3*ntargets). */
+void orc_compute_velocities_rows(int analytic, double eps, int N, int M,
+                                 const double *pos, const double *orient, const double *force,
+                                 int LQ, const double *pv, const double *wv, const double *lvec,
+                                 const double *coeff, int ntargets, const int *targets,
+                                 double *vel, double *rot);
 
 /* LU with partial pivoting; overwrites A with factors and b with x.
  * Stands in for LAPACK dgesv (fortran/coeffs_and_vels.f90:62,158).

@@ -1,0 +1,307 @@
+"""MI355X-native rigid-fiber slender-body timestep (Python host mirror).
+
+The compute path is ``librigidfibers.so`` (HIP for gfx950, C ABI declared in
+``include/rigidfibers.h``).  This module binds it with ctypes and mirrors the
+reference's host interface:
+
+* ``Parameters.parse_configuration_files(parameters, layout)`` --
+  src/parameters.cc:24-43 (layout first, then the v1 parameter file);
+* ``Simulation(configuration)`` / ``Simulation.step()`` --
+  src/simulation.h:36-87, src/simulation.cu:279-301.
+
+There is no CPU fallback: every call goes to the HIP library, and importing
+this package raises if the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librigidfibers.so")
+
+RF_OK = 0
+RF_EINVAL, RF_EIO, RF_ENOMEM, RF_EHIP, RF_ENOCONV = -1, -2, -3, -4, -5
+PREC_FP64, PREC_MIXED = 0, 1
+WARM_PREV, WARM_ZERO, WARM_EXTRAP, WARM_LEARNED = 0, 1, 2, 3
+FMT_F16_8, FMT_F24_16 = 0, 1
+SOLVER_GMRES, SOLVER_LU, SOLVER_LU_FALLBACK = 0, 1, 2
+
+
+class rf_params(ctypes.Structure):
+    _fields_ = [
+        ("run_label", ctypes.c_int32),
+        ("layout_label", ctypes.c_int32),
+        ("num_terms", ctypes.c_int32),
+        ("slenderness", ctypes.c_double),
+        ("timestep", ctypes.c_double),
+        ("num_timesteps", ctypes.c_int32),
+        ("save_interval", ctypes.c_int32),
+        ("saves_per_file", ctypes.c_int32),
+        ("num_quad_intervals", ctypes.c_int32),
+        ("num_quad_points_per_interval", ctypes.c_int32),
+        ("use_analytical_integration", ctypes.c_int32),
+        ("use_direct_solver", ctypes.c_int32),
+        ("gmres_restart", ctypes.c_int32),
+        ("gmres_max_iters", ctypes.c_int32),
+        ("gmres_tol", ctypes.c_double),
+        ("external_force", ctypes.c_double * 3),
+        ("precision", ctypes.c_int32),
+        ("warm_start", ctypes.c_int32),
+        ("profile", ctypes.c_int32),
+        ("lu_block", ctypes.c_int32),
+        ("mixed_inner_tol", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class rf_stats(ctypes.Structure):
+    _fields_ = [
+        ("steps_taken", ctypes.c_int64),
+        ("last_gmres_iters", ctypes.c_int32),
+        ("last_solver", ctypes.c_int32),
+        ("last_backward_error", ctypes.c_double),
+        ("total_gmres_iters", ctypes.c_int64),
+        ("ms_assemble", ctypes.c_double),
+        ("ms_solve", ctypes.c_double),
+        ("ms_velocities", ctypes.c_double),
+        ("ms_advance", ctypes.c_double),
+        ("total_ms_assemble", ctypes.c_double),
+        ("total_ms_solve", ctypes.c_double),
+        ("total_ms_velocities", ctypes.c_double),
+        ("total_ms_advance", ctypes.c_double),
+        ("gemv_launches", ctypes.c_int64),
+        ("gemv_ms_total", ctypes.c_double),
+        ("assemble_launches", ctypes.c_int64),
+        ("assemble_ms_total", ctypes.c_double),
+        ("velocity_gemv_launches", ctypes.c_int64),
+        ("velocity_gemv_ms_total", ctypes.c_double),
+        ("lu_factorizations", ctypes.c_int64),
+        ("lu_ms_total", ctypes.c_double),
+        ("matrix_bytes", ctypes.c_double),
+        ("matrix_elem_bytes", ctypes.c_int32),
+        ("lda", ctypes.c_int32),
+        ("reserved", ctypes.c_double * 8),
+    ]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "reserved"}
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class RigidFibersError(RuntimeError):
+    def __init__(self, code, where):
+        msg = lib().rf_last_error().decode(errors="replace")
+        super().__init__("%s failed (%d): %s" % (where, code, msg))
+        self.code = code
+
+
+# exported symbols and their argument types (include/rigidfibers.h)
+_SIGNATURES = {
+    "rf_params_default": (None, [ctypes.POINTER(rf_params)]),
+    "rf_parse_parameters": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(rf_params)]),
+    "rf_parse_layout": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(_dp), ctypes.POINTER(_dp)]),
+    "rf_free": (None, [ctypes.c_void_p]),
+    "rf_create": (ctypes.c_int, [ctypes.POINTER(rf_params), ctypes.c_int32, _dp, _dp,
+                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "rf_destroy": (None, [ctypes.c_void_p]),
+    "rf_step": (ctypes.c_int, [ctypes.c_void_p]),
+    "rf_assemble": (ctypes.c_int, [ctypes.c_void_p]),
+    "rf_solve": (ctypes.c_int, [ctypes.c_void_p]),
+    "rf_update_velocities": (ctypes.c_int, [ctypes.c_void_p]),
+    "rf_advance": (ctypes.c_int, [ctypes.c_void_p]),
+    "rf_get_state": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "rf_get_velocities": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "rf_get_coefficients": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "rf_get_rhs": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "rf_get_matrix": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int64]),
+    "rf_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(rf_stats)]),
+    "rf_set_coefficients": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "rf_matvec": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "rf_write_step_outputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]),
+    "rf_num_fibers": (ctypes.c_int32, [ctypes.c_void_p]),
+    "rf_num_unknowns": (ctypes.c_int64, [ctypes.c_void_p]),
+    "rf_last_error": (ctypes.c_char_p, []),
+    "rf_version": (ctypes.c_char_p, []),
+}
+
+
+def lib():
+    """Loads librigidfibers.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("librigidfibers.so is not built (run __graft_entry__.build() or "
+                              "make -C rigidfibers_kth_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def _check(rc, where):
+    if rc != RF_OK:
+        raise RigidFibersError(rc, where)
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def default_params(**kw):
+    p = rf_params()
+    lib().rf_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        if k == "external_force":
+            for i in range(3):
+                p.external_force[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Configuration:
+    """FiberParams + initial layout (src/parameters.h:27-31)."""
+
+    def __init__(self, parameters, positions, orientations):
+        self.parameters = parameters
+        self.initial_positions = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+        self.initial_orientations = np.ascontiguousarray(orientations, dtype=np.float64).reshape(-1)
+
+    @property
+    def num_fibers(self):
+        return self.initial_positions.size // 3
+
+
+class Parameters:
+    """src/parameters.{h,cc}: v1 parameter and layout files."""
+
+    @staticmethod
+    def parse_parameter_file(path):
+        p = rf_params()
+        _check(lib().rf_parse_parameters(os.fsencode(path), ctypes.byref(p)), "rf_parse_parameters")
+        return p
+
+    @staticmethod
+    def parse_initial_layout_file(path):
+        n = ctypes.c_int32(0)
+        pos = _dp()
+        ori = _dp()
+        _check(lib().rf_parse_layout(os.fsencode(path), ctypes.byref(n), ctypes.byref(pos), ctypes.byref(ori)),
+               "rf_parse_layout")
+        N = n.value
+        P = np.ctypeslib.as_array(pos, shape=(3 * N,)).copy()
+        T = np.ctypeslib.as_array(ori, shape=(3 * N,)).copy()
+        lib().rf_free(ctypes.cast(pos, ctypes.c_void_p))
+        lib().rf_free(ctypes.cast(ori, ctypes.c_void_p))
+        return P, T
+
+    @staticmethod
+    def parse_configuration_files(parameters_filename, layout_filename):
+        pos, ori = Parameters.parse_initial_layout_file(layout_filename)
+        params = Parameters.parse_parameter_file(parameters_filename)
+        params.num_quad_points_per_interval = 3
+        return Configuration(params, pos, ori)
+
+
+class Simulation:
+    """One rf_sim handle (src/simulation.h:36-87)."""
+
+    def __init__(self, configuration):
+        self.configuration = configuration
+        self._h = ctypes.c_void_p()
+        _check(lib().rf_create(ctypes.byref(configuration.parameters), configuration.num_fibers,
+                               _p(configuration.initial_positions), _p(configuration.initial_orientations),
+                               ctypes.byref(self._h)), "rf_create")
+        self.N = lib().rf_num_fibers(self._h)
+        self.n = lib().rf_num_unknowns(self._h)
+
+    def close(self):
+        if self._h:
+            lib().rf_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- phases (Simulation::step, src/simulation.cu:279-301) ---
+    def step(self, current_timestep=None):
+        _check(lib().rf_step(self._h), "rf_step")
+
+    def assemble_system(self):
+        _check(lib().rf_assemble(self._h), "rf_assemble")
+
+    def solve_system(self):
+        _check(lib().rf_solve(self._h), "rf_solve")
+
+    def update_velocities(self):
+        _check(lib().rf_update_velocities(self._h), "rf_update_velocities")
+
+    def update_fibers(self):
+        _check(lib().rf_advance(self._h), "rf_advance")
+
+    # --- state ---
+    def state(self):
+        pos = np.empty(3 * self.N)
+        ori = np.empty(3 * self.N)
+        _check(lib().rf_get_state(self._h, _p(pos), _p(ori)), "rf_get_state")
+        return pos, ori
+
+    def velocities(self):
+        v = np.empty(3 * self.N)
+        w = np.empty(3 * self.N)
+        _check(lib().rf_get_velocities(self._h, _p(v), _p(w)), "rf_get_velocities")
+        return v, w
+
+    def coefficients(self):
+        x = np.empty(self.n)
+        _check(lib().rf_get_coefficients(self._h, _p(x)), "rf_get_coefficients")
+        return x
+
+    def set_coefficients(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        _check(lib().rf_set_coefficients(self._h, _p(x)), "rf_set_coefficients")
+
+    def rhs(self):
+        b = np.empty(self.n)
+        _check(lib().rf_get_rhs(self._h, _p(b)), "rf_get_rhs")
+        return b
+
+    def matrix(self):
+        A = np.empty((self.n, self.n), order="F")
+        _check(lib().rf_get_matrix(self._h, A.ctypes.data_as(_dp), self.n), "rf_get_matrix")
+        return A
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        _check(lib().rf_matvec(self._h, _p(x), _p(y)), "rf_matvec")
+        return y
+
+    def stats(self):
+        s = rf_stats()
+        _check(lib().rf_get_stats(self._h, ctypes.byref(s)), "rf_get_stats")
+        return s
+
+    def write_step_outputs(self, directory, fmt=FMT_F16_8):
+        _check(lib().rf_write_step_outputs(self._h, os.fsencode(directory), fmt), "rf_write_step_outputs")

@@ -1,0 +1,446 @@
+// rf_assemble.hip -- pairwise assembly of the rigid-fiber linear system on
+// gfx950.
+//
+// For every ordered pair (target fiber i, source fiber j != i) the kernel
+// computes the outer integrals
+//     Theta^{ij}_{kl} = sum_p w_p P_k(s_p) G_l(x_i + s_p t_i),  k,l = 0..M,
+// where G_l is the inner integral over fiber j of the Stokeslet+doublet
+// kernel times P_l (3x3 symmetric, 6 components):
+//   analytic: fortran/Int_analytical.f90:5-293,663-767 (moments I,J,S with
+//             forward recurrence for c<10, backward from n=27 otherwise);
+//   numeric : fortran/quadrature.f90:100-164 (LQ-point Gauss quadrature).
+// From Theta it writes
+//   * A (column-major): rows (i,k,a), k=1..M, cols (j,l,b), l=1..M,
+//       k=1: D1 t_a (Theta_kl t)_b ;  k>=2: gamma_k (Theta_kl,ab - E_k t_a (Theta_kl t)_b)
+//     (fortran/matrix_assembly.f90:341-395), identity on the diagonal blocks;
+//   * partial right-hand sides from Theta_{k0} f0_j
+//     (fortran/matrix_assembly.f90:632-905), one slice per source split;
+//   * the velocity operator W: rows (i, r in {V,Omega}, c), cols (j,l,c'),
+//     W = Theta_{r l} (P_0 and P_1 outer weights of
+//     fortran/coeffs_and_vels.f90:612-620,715-724), so the velocities of the
+//     step are one GEMV with [F/2, a^1..a^M] per source fiber.
+//
+// Work decomposition: a workgroup owns T consecutive target fibers and one
+// range of source fibers.  Phase 1: thread (t,p) evaluates G_0..G_M at the
+// quadrature point p of target t and stores them in LDS.  Phase 2: thread
+// (t,l,c) contracts over p into Theta_{0..M,l,c}.  Phase 3: threads own rows
+// of A / W and write the 3M (resp. 3(M+1)) columns of source j, so each
+// column segment of T*3M consecutive rows is one coalesced store.
+#include "rf_internal.h"
+
+#include <algorithm>
+
+namespace rf {
+namespace {
+
+// Coefficients of P_l(s) = sum_m c[l][m] s^m (the combinations of
+// fortran/Int_analytical.f90:213-288).
+__device__ constexpr double kLeg[9][9] = {
+    {1, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 1, 0, 0, 0, 0, 0, 0, 0},
+    {-0.5, 0, 1.5, 0, 0, 0, 0, 0, 0},
+    {0, -1.5, 0, 2.5, 0, 0, 0, 0, 0},
+    {0.375, 0, -3.75, 0, 4.375, 0, 0, 0, 0},
+    {0, 1.875, 0, -8.75, 0, 7.875, 0, 0, 0},
+    {-0.3125, 0, 6.5625, 0, -19.6875, 0, 14.4375, 0, 0},
+    {0, -2.1875, 0, 19.6875, 0, -43.3125, 0, 26.8125, 0},
+    {0.2734375, 0, -9.84375, 0, 54.140625, 0, -93.84375, 0, 50.2734375},
+};
+
+// Symmetric 3x3 <-> 6-vector (11,22,33,12,13,23).
+__device__ __forceinline__ int sym6(int a, int b)
+{
+    return (a == b) ? a : (a + b + 2);  // (0,1)->3 (0,2)->4 (1,2)->5
+}
+
+// Analytic inner integrals G_0..G_M at point X against fiber (xj,tj).
+// R is the arithmetic type of the moment recurrences (double, or float for
+// the far field in mixed precision).
+template <int M, typename R>
+__device__ __forceinline__ void analytic_G(double X0, double X1, double X2, const double* xj,
+                                           const double* tj, double eps, double (&G)[(M + 1) * 6])
+{
+    constexpr int NM = M + 3;  // moments 0..M+2
+    const R r0x = (R)(X0 - xj[0]), r0y = (R)(X1 - xj[1]), r0z = (R)(X2 - xj[2]);
+    const R px = (R)tj[0], py = (R)tj[1], pz = (R)tj[2];
+    const R b = R(-2) * (px * r0x + py * r0y + pz * r0z);
+    const R c = r0x * r0x + r0y * r0y + r0z * r0z;
+    const R ue = sqrt(R(1) + b + c);
+    const R ub = sqrt(R(1) - b + c);
+    const R d = c - R(0.25) * b * b;
+    R I[NM], J[NM], S[NM];
+    I[0] = log(fabs(R(2) + b + R(2) * ue)) - log(fabs(R(-2) + b + R(2) * ub));
+    I[1] = ue - ub - R(0.5) * b * I[0];
+    const R iue = R(1) / ue, iub = R(1) / ub;
+    if (d < R(1e-14)) {  // collinear (Int_analytical.f90:162-169,189-196)
+        const R sp = R(1) + R(0.5) * b, sm = R(-1) + R(0.5) * b;
+        J[0] = R(-0.5) / (sp * sp) + R(0.5) / (sm * sm);
+        J[1] = -iue + iub - R(0.5) * b * J[0];
+        const R ap = R(2) + b, am = R(-2) + b;
+        S[0] = R(-4) / (ap * ap * ap * ap) + R(4) / (am * am * am * am);
+        S[1] = R(-8) / (R(3) * ap * ap * ap) + R(8) / (R(3) * am * am * am) - R(0.5) * b * S[0];
+    } else {
+        const R id = R(1) / d;
+        J[0] = ((R(1) + R(0.5) * b) * iue - (R(-1) + R(0.5) * b) * iub) * id;
+        J[1] = -iue + iub - R(0.5) * b * J[0];
+        const R iue3 = iue * iue * iue, iub3 = iub * iub * iub;
+        const R i6d = id * R(1.0 / 6.0);
+        S[0] = ((R(2) + b) * iue3 - (R(-2) + b) * iub3) * i6d + (R(2.0 / 3.0) * id) * J[0];
+        S[1] = (-(b + R(2) * c) * iue3 + (R(2) * c - b) * iub3) * i6d - (b * id * R(1.0 / 3.0)) * J[0];
+    }
+    if (c < R(10)) {
+        // forward recurrences (Int_analytical.f90:123-136,172-176,197-200)
+#pragma unroll
+        for (int n = 2; n < NM; ++n) {
+            const R sgn = (n % 2 == 1) ? R(1) : R(-1);  // (-1)^(n-1)
+            I[n] = (ue - sgn * ub) * R(1.0 / n) - R((n - 0.5) / n) * b * I[n - 1] - R((n - 1.0) / n) * c * I[n - 2];
+            J[n] = I[n - 2] - b * J[n - 1] - c * J[n - 2];
+            S[n] = J[n - 2] - b * S[n - 1] - c * S[n - 2];
+        }
+    } else {
+        // backward recurrences from n = 27 with zero start
+        // (Int_analytical.f90:137-158,177-183,201-206)
+        const R ic = R(1) / c;
+        R i1 = 0, i2 = 0, j1 = 0, j2 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int m = 27; m >= 2; --m) {
+            const R sgn = ((m + 1) % 2 == 0) ? R(1) : R(-1);  // (-1)^(m+1)
+            const R im = ic * R(1.0 / (m + 1.0)) * ((ue - sgn * ub) - R(m + 1.5) * b * i1 - R(m + 2.0) * i2);
+            const R jm = ic * (im - b * j1 - j2);
+            const R sm = ic * (jm - b * s1 - s2);
+            if (m < NM) {
+                I[m] = im;
+                J[m] = jm;
+                S[m] = sm;
+            }
+            i2 = i1;
+            i1 = im;
+            j2 = j1;
+            j1 = jm;
+            s2 = s1;
+            s1 = sm;
+        }
+    }
+    const R e2 = R(2.0 * eps * eps), e6 = R(6.0 * eps * eps);
+#pragma unroll
+    for (int l = 0; l <= M; ++l) {
+        R L11 = 0, L12 = 0, L13 = 0, L14 = 0, L22 = 0, L23 = 0, L24 = 0;
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+            if (kLeg[l][m] != 0.0) {
+                const R cf = R(kLeg[l][m]);
+                L11 += cf * I[m];
+                L12 += cf * J[m];
+                L13 += cf * J[m + 1];
+                L14 += cf * J[m + 2];
+                L22 += cf * S[m];
+                L23 += cf * S[m + 1];
+                L24 += cf * S[m + 2];
+            }
+        }
+        // G = al I + be R0 R0^T + ga (t R0^T + R0 t^T) + de t t^T
+        // (Int_analytical.f90:735-736 regrouped)
+        const R al = L11 + e2 * L12;
+        const R be = L12 - e6 * L22;
+        const R ga = e6 * L23 - L13;
+        const R de = L14 - e6 * L24;
+        G[l * 6 + 0] = (double)(al + be * r0x * r0x + R(2) * ga * px * r0x + de * px * px);
+        G[l * 6 + 1] = (double)(al + be * r0y * r0y + R(2) * ga * py * r0y + de * py * py);
+        G[l * 6 + 2] = (double)(al + be * r0z * r0z + R(2) * ga * pz * r0z + de * pz * pz);
+        G[l * 6 + 3] = (double)(be * r0x * r0y + ga * (px * r0y + r0x * py) + de * px * py);
+        G[l * 6 + 4] = (double)(be * r0x * r0z + ga * (px * r0z + r0x * pz) + de * px * pz);
+        G[l * 6 + 5] = (double)(be * r0y * r0z + ga * (py * r0z + r0y * pz) + de * py * pz);
+    }
+}
+
+// Numeric inner integrals (fortran/quadrature.f90:121-155): LQ-point sum of
+// the kernel times w_q P_l(s_q) over fiber j.
+template <int M, typename R>
+__device__ __forceinline__ void numeric_G(double X0, double X1, double X2, const double* xj, const double* tj,
+                                          double eps, int LQ, const double* __restrict__ pv,
+                                          const double* __restrict__ wp, double (&G)[(M + 1) * 6])
+{
+    R acc[(M + 1) * 6];
+#pragma unroll
+    for (int q = 0; q < (M + 1) * 6; ++q) acc[q] = 0;
+    const R e2 = R(2.0 * eps * eps), e6 = R(6.0 * eps * eps);
+    const double bx = X0 - xj[0], by = X1 - xj[1], bz = X2 - xj[2];
+    for (int q = 0; q < LQ; ++q) {
+        const double s = pv[q];
+        const R rx = (R)(bx - tj[0] * s), ry = (R)(by - tj[1] * s), rz = (R)(bz - tj[2] * s);
+        const R r2 = rx * rx + ry * ry + rz * rz;
+        const R inv = R(1) / sqrt(r2);
+        const R inv2 = inv * inv, inv3 = inv * inv2, inv5 = inv3 * inv2;
+        const R dg = inv + e2 * inv3, of = inv3 - e6 * inv5;
+        const R K0 = dg + of * rx * rx, K1 = dg + of * ry * ry, K2 = dg + of * rz * rz;
+        const R K3 = of * rx * ry, K4 = of * rx * rz, K5 = of * ry * rz;
+#pragma unroll
+        for (int l = 0; l <= M; ++l) {
+            const R w = (R)wp[l * LQ + q];
+            acc[l * 6 + 0] += w * K0;
+            acc[l * 6 + 1] += w * K1;
+            acc[l * 6 + 2] += w * K2;
+            acc[l * 6 + 3] += w * K3;
+            acc[l * 6 + 4] += w * K4;
+            acc[l * 6 + 5] += w * K5;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < (M + 1) * 6; ++q) G[q] = (double)acc[q];
+}
+
+struct KArgs {
+    int N, n, LQ, T, src_per_split;
+    int64_t lda, ldw;
+    int analytic, rhs_z_only;
+    double eps, D1, near_c;
+    double Ek[kMaxTerms + 1], gam[kMaxTerms + 1];
+    const double* pos;
+    const double* orient;
+    const double* force;
+    const double* wp;
+    const double* pv;
+    void* A;
+    void* W;
+    double* bpart;
+};
+
+constexpr int kMaxItems = 4;  // phase-3 row items per thread
+
+template <int M, bool ANALYTIC, bool MIXED>
+__global__ void __launch_bounds__(256) assemble_kernel(const KArgs a)
+{
+    using AT = typename std::conditional<MIXED, float, double>::type;
+    constexpr int NL = M + 1;
+    extern __shared__ double smem[];
+    const int T = a.T, LQ = a.LQ, N = a.N;
+    const int gstride = LQ + 1;
+    double* sG = smem;                              // [T*NL*6][LQ+1]
+    double* sTh = smem + (size_t)T * NL * 6 * gstride;  // [T][NL k][NL l][6]
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int i0 = blockIdx.x * T;
+    const int jb = blockIdx.y * a.src_per_split;
+    const int je = min(N, jb + a.src_per_split);
+
+    // phase-1 identity: target t1, point p1
+    const int t1 = tid / LQ, p1 = tid - (tid / LQ) * LQ;
+    const int i1 = i0 + t1;
+    const bool on1 = (t1 < T) && (i1 < N);
+    double X0 = 0, X1 = 0, X2 = 0;
+    if (on1) {
+        const double s = a.pv[p1];
+        X0 = a.pos[3 * i1 + 0] + s * a.orient[3 * i1 + 0];
+        X1 = a.pos[3 * i1 + 1] + s * a.orient[3 * i1 + 1];
+        X2 = a.pos[3 * i1 + 2] + s * a.orient[3 * i1 + 2];
+    }
+    const int nA = T * 3 * M;          // A-row items
+    const int nItems = nA + T * 6;     // + W-row items
+    double bacc[kMaxItems];
+#pragma unroll
+    for (int s = 0; s < kMaxItems; ++s) bacc[s] = 0.0;
+
+    for (int j = jb; j < je; ++j) {
+        const double xj[3] = {a.pos[3 * j], a.pos[3 * j + 1], a.pos[3 * j + 2]};
+        const double tj[3] = {a.orient[3 * j], a.orient[3 * j + 1], a.orient[3 * j + 2]};
+        // ---- phase 1: G_0..G_M at (t1, p1) ----
+        if (t1 < T) {
+            double G[NL * 6];
+            if (on1 && i1 != j) {
+                bool fp64 = true;
+                if (MIXED) {
+                    const double dx = X0 - xj[0], dy = X1 - xj[1], dz = X2 - xj[2];
+                    fp64 = (dx * dx + dy * dy + dz * dz) < a.near_c;
+                }
+                if (ANALYTIC) {
+                    if (fp64)
+                        analytic_G<M, double>(X0, X1, X2, xj, tj, a.eps, G);
+                    else
+                        analytic_G<M, float>(X0, X1, X2, xj, tj, a.eps, G);
+                } else {
+                    if (fp64)
+                        numeric_G<M, double>(X0, X1, X2, xj, tj, a.eps, LQ, a.pv, a.wp, G);
+                    else
+                        numeric_G<M, float>(X0, X1, X2, xj, tj, a.eps, LQ, a.pv, a.wp, G);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NL * 6; ++q) G[q] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < NL * 6; ++q) sG[(size_t)(t1 * NL * 6 + q) * gstride + p1] = G[q];
+        }
+        __syncthreads();
+        // ---- phase 2: Theta_{k l c} = sum_p w_p P_k(s_p) G_{l c}(p) ----
+        for (int it = tid; it < T * NL * 6; it += nthr) {
+            const double* g = sG + (size_t)it * gstride;
+            double acc[NL];
+#pragma unroll
+            for (int k = 0; k < NL; ++k) acc[k] = 0.0;
+            for (int p = 0; p < LQ; ++p) {
+                const double gp = g[p];
+#pragma unroll
+                for (int k = 0; k < NL; ++k) acc[k] += a.wp[k * LQ + p] * gp;
+            }
+            const int t = it / (NL * 6), lc = it - t * (NL * 6);
+            const int l = lc / 6, c = lc - l * 6;
+#pragma unroll
+            for (int k = 0; k < NL; ++k) sTh[((t * NL + k) * NL + l) * 6 + c] = acc[k];
+        }
+        __syncthreads();
+        // ---- phase 3: A rows, W rows, rhs partials ----
+        int slot = 0;
+        for (int it = tid; it < nItems; it += nthr, ++slot) {
+            if (it < nA) {
+                const int t = it / (3 * M), rem = it - t * (3 * M);
+                const int i = i0 + t;
+                if (i >= N) continue;
+                const int k = rem / 3 + 1, ar = rem - (rem / 3) * 3;
+                const double ti[3] = {a.orient[3 * i], a.orient[3 * i + 1], a.orient[3 * i + 2]};
+                const int64_t row = (int64_t)i * 3 * M + rem;
+                AT* A = reinterpret_cast<AT*>(a.A);
+                if (i == j) {
+#pragma unroll
+                    for (int l = 1; l <= M; ++l)
+#pragma unroll
+                        for (int bc = 0; bc < 3; ++bc) {
+                            const int64_t col = (int64_t)j * 3 * M + (l - 1) * 3 + bc;
+                            A[col * a.lda + row] = (AT)((k == l && ar == bc) ? 1.0 : 0.0);
+                        }
+                    continue;
+                }
+                const double* th = sTh + (size_t)(t * NL + k) * NL * 6;
+                const double Ek = a.Ek[k], gk = a.gam[k];
+#pragma unroll
+                for (int l = 1; l <= M; ++l) {
+                    const double* tl = th + l * 6;
+#pragma unroll
+                    for (int bc = 0; bc < 3; ++bc) {
+                        const double Qb = tl[sym6(bc, 0)] * ti[0] + tl[sym6(bc, 1)] * ti[1] + tl[sym6(bc, 2)] * ti[2];
+                        const double v = (k == 1) ? a.D1 * ti[ar] * Qb : gk * (tl[sym6(ar, bc)] - Ek * ti[ar] * Qb);
+                        const int64_t col = (int64_t)j * 3 * M + (l - 1) * 3 + bc;
+                        A[col * a.lda + row] = (AT)v;
+                    }
+                }
+                // rhs: contr = Theta_{k0} f0_j
+                const double F0 = a.rhs_z_only ? 0.0 : 0.5 * a.force[3 * j + 0];
+                const double F1 = a.rhs_z_only ? 0.0 : 0.5 * a.force[3 * j + 1];
+                const double F2 = 0.5 * a.force[3 * j + 2];
+                const double* t0 = th;  // l = 0
+                double cr[3];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+                    cr[cc] = t0[sym6(cc, 0)] * F0 + t0[sym6(cc, 1)] * F1 + t0[sym6(cc, 2)] * F2;
+                const double tdc = ti[0] * cr[0] + ti[1] * cr[1] + ti[2] * cr[2];
+                const double db = (k == 1) ? a.D1 * tdc * ti[ar] : gk * (cr[ar] - Ek * tdc * ti[ar]);
+#pragma unroll
+                for (int s = 0; s < kMaxItems; ++s)
+                    if (s == slot) bacc[s] -= db;
+            } else {
+                const int u = it - nA;
+                const int t = u / 6, r6 = u - t * 6;
+                const int i = i0 + t;
+                if (i >= N) continue;
+                const int r = r6 / 3, c = r6 - r * 3;
+                const int64_t row = (int64_t)i * 6 + r6;
+                AT* W = reinterpret_cast<AT*>(a.W);
+                const double* th = sTh + (size_t)(t * NL + r) * NL * 6;
+#pragma unroll
+                for (int l = 0; l <= M; ++l)
+#pragma unroll
+                    for (int c2 = 0; c2 < 3; ++c2) {
+                        const int64_t col = (int64_t)j * 3 * NL + l * 3 + c2;
+                        W[col * a.ldw + row] = (AT)((i == j) ? 0.0 : th[l * 6 + sym6(c, c2)]);
+                    }
+            }
+        }
+    }
+    // rhs partial of this source split
+    int slot = 0;
+    for (int it = tid; it < nA; it += nthr, ++slot) {
+        const int t = it / (3 * M), rem = it - t * (3 * M);
+        const int i = i0 + t;
+        if (i >= N) continue;
+        double v = 0.0;
+#pragma unroll
+        for (int s = 0; s < kMaxItems; ++s)
+            if (s == slot) v = bacc[s];
+        a.bpart[(size_t)blockIdx.y * a.n + (size_t)i * 3 * M + rem] = v;
+    }
+}
+
+template <int M>
+hipError_t launch_m(const KArgs& k, dim3 grid, int threads, size_t lds, bool analytic, bool mixed,
+                    hipStream_t s)
+{
+    if (analytic && !mixed)
+        hipLaunchKernelGGL((assemble_kernel<M, true, false>), grid, dim3(threads), lds, s, k);
+    else if (analytic && mixed)
+        hipLaunchKernelGGL((assemble_kernel<M, true, true>), grid, dim3(threads), lds, s, k);
+    else if (!analytic && !mixed)
+        hipLaunchKernelGGL((assemble_kernel<M, false, false>), grid, dim3(threads), lds, s, k);
+    else
+        hipLaunchKernelGGL((assemble_kernel<M, false, true>), grid, dim3(threads), lds, s, k);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int assemble_targets_per_block(int LQ)
+{
+    // T*LQ threads; prefer a whole number of 64-wide wavefronts <= 256.
+    for (int thr = 256; thr >= 64; thr -= 64)
+        if (thr % LQ == 0) return thr / LQ;
+    return std::max(1, 256 / LQ);
+}
+
+hipError_t launch_assemble(const Tables& t, const AssembleArgs& a, hipStream_t s)
+{
+    const int M = t.M, LQ = t.LQ;
+    const int T = a.targets_per_block > 0 ? a.targets_per_block : assemble_targets_per_block(LQ);
+    const int threads = T * LQ;
+    if (threads > 1024 || T * (3 * M + 6) > kMaxItems * threads) return hipErrorInvalidValue;
+    KArgs k;
+    k.N = a.N;
+    k.n = a.n;
+    k.LQ = LQ;
+    k.T = T;
+    k.src_per_split = (a.N + a.splits - 1) / a.splits;
+    k.lda = a.lda;
+    k.ldw = a.ldw;
+    k.analytic = a.analytic;
+    k.rhs_z_only = a.analytic ? a.rhs_z_only : 0;
+    k.eps = t.eps;
+    k.D1 = t.D1;
+    k.near_c = a.near_c;
+    for (int i = 0; i <= kMaxTerms; ++i) {
+        k.Ek[i] = t.Ek[i];
+        k.gam[i] = t.gammak[i];
+    }
+    k.pos = a.pos;
+    k.orient = a.orient;
+    k.force = a.force;
+    k.wp = a.wp;
+    k.pv = a.pv;
+    k.A = a.A;
+    k.W = a.W;
+    k.bpart = a.bpart;
+    const int NL = M + 1;
+    const size_t lds = sizeof(double) * ((size_t)T * NL * 6 * (LQ + 1) + (size_t)T * NL * NL * 6);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    dim3 grid((a.N + T - 1) / T, a.splits);
+    const bool an = a.analytic != 0, mx = a.mixed != 0;
+    switch (M) {
+    case 1: return launch_m<1>(k, grid, threads, lds, an, mx, s);
+    case 2: return launch_m<2>(k, grid, threads, lds, an, mx, s);
+    case 3: return launch_m<3>(k, grid, threads, lds, an, mx, s);
+    case 4: return launch_m<4>(k, grid, threads, lds, an, mx, s);
+    case 5: return launch_m<5>(k, grid, threads, lds, an, mx, s);
+    case 6: return launch_m<6>(k, grid, threads, lds, an, mx, s);
+    case 7: return launch_m<7>(k, grid, threads, lds, an, mx, s);
+    case 8: return launch_m<8>(k, grid, threads, lds, an, mx, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace rf

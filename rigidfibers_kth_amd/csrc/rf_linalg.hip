@@ -1,0 +1,292 @@
+// rf_linalg.hip -- HBM-bound vector kernels of the solver on gfx950:
+// column-major GEMV (the GMRES matvec of fortran/coeffs_and_vels.f90:299,483
+// and the velocity-operator product), deterministic multi-dot reductions and
+// Gram-Schmidt updates of the Arnoldi basis.
+//
+// All arithmetic is FP64.  The matrix may be stored as double or float; the
+// float case (mixed precision) halves the bytes streamed from HBM while the
+// products and sums stay FP64.
+#include "rf_internal.h"
+
+#include <algorithm>
+
+namespace rf {
+namespace {
+
+constexpr int kGemvThreads = 256;  // 4 wavefronts per workgroup
+constexpr int kGemvUnroll = 8;     // columns in flight per wavefront
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+template <>
+struct Vec16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+
+__device__ __forceinline__ void unpack(const double2& v, double (&o)[2])
+{
+    o[0] = v.x;
+    o[1] = v.y;
+}
+__device__ __forceinline__ void unpack(const float4& v, double (&o)[4])
+{
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+
+// parts[split][row] = sum_{c in split} A[row + c*ld] * x[c]
+// Workgroup: 64*RPL consecutive rows x one column split.  Wavefront w of the
+// workgroup takes columns c0+w, c0+w+4, ...; each lane loads 16 bytes (RPL
+// rows) per column, so one wavefront load is a 1 KiB contiguous segment.
+template <typename T>
+__global__ void __launch_bounds__(kGemvThreads) gemv_parts_kernel(int cols, int64_t ld, const T* __restrict__ A,
+                                                                  const double* __restrict__ x,
+                                                                  double* __restrict__ parts, int64_t pstride,
+                                                                  int cols_per_split)
+{
+    using V = typename Vec16<T>::type;
+    constexpr int RPL = Vec16<T>::n;
+    constexpr int ROWS = 64 * RPL;
+    __shared__ double red[4][ROWS];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t row0 = (int64_t)blockIdx.x * ROWS + lane * RPL;
+    const int c0 = blockIdx.y * cols_per_split;
+    const int c1 = min(cols, c0 + cols_per_split);
+    double acc[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) acc[r] = 0.0;
+    const T* Ab = A + row0;
+    int c = c0 + wave;
+    for (; c + 4 * (kGemvUnroll - 1) < c1; c += 4 * kGemvUnroll) {
+        V v[kGemvUnroll];
+        double xv[kGemvUnroll];
+#pragma unroll
+        for (int u = 0; u < kGemvUnroll; ++u) {
+            v[u] = *reinterpret_cast<const V*>(Ab + (int64_t)(c + 4 * u) * ld);
+            xv[u] = x[c + 4 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < kGemvUnroll; ++u) {
+            double e[RPL];
+            unpack(v[u], e);
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) acc[r] += e[r] * xv[u];
+        }
+    }
+    for (; c < c1; c += 4) {
+        const V v = *reinterpret_cast<const V*>(Ab + (int64_t)c * ld);
+        const double xv = x[c];
+        double e[RPL];
+        unpack(v, e);
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) acc[r] += e[r] * xv;
+    }
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) red[wave][lane * RPL + r] = acc[r];
+    __syncthreads();
+    for (int rr = threadIdx.x; rr < ROWS; rr += kGemvThreads) {
+        const double s = (red[0][rr] + red[1][rr]) + (red[2][rr] + red[3][rr]);
+        parts[(int64_t)blockIdx.y * pstride + (int64_t)blockIdx.x * ROWS + rr] = s;
+    }
+}
+
+// y[i] = sum_s parts[s][i]  (mode 0)  or  b[i] - sum_s parts[s][i]  (mode 1)
+__global__ void reduce_splits_kernel(const double* __restrict__ parts, int splits, int64_t stride, int n,
+                                     const double* __restrict__ b, int mode, double* __restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < splits; ++k) s += parts[(int64_t)k * stride + i];
+    y[i] = (mode == 1) ? b[i] - s : s;
+}
+
+constexpr int kDotThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+    }
+    __syncthreads();
+    return s;  // valid in thread 0
+}
+
+// Block (bx, j): partial of dot(V[:,j], w) (j < k) or dot(w,w) (j == k) over a
+// row chunk.  The last workgroup to finish sums the partials in a fixed order
+// (deterministic) into out[0..k] and resets the counter.
+__global__ void __launch_bounds__(kDotThreads) multidot_kernel(const double* __restrict__ V, int64_t ldv, int k,
+                                                               const double* __restrict__ w, int n,
+                                                               double* __restrict__ parts,
+                                                               unsigned int* counter, double* __restrict__ out)
+{
+    __shared__ double sh[kDotThreads / 64];
+    __shared__ bool last;
+    const int j = blockIdx.y;
+    const int nbx = gridDim.x;
+    const double* a = (j < k) ? V + (int64_t)j * ldv : w;
+    double s = 0.0;
+    for (int i = blockIdx.x * kDotThreads + threadIdx.x; i < n; i += nbx * kDotThreads) s += a[i] * w[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        parts[(int64_t)j * nbx + blockIdx.x] = s;
+        __threadfence();
+        const unsigned int total = gridDim.x * gridDim.y;
+        const unsigned int prev = atomicAdd(counter, 1u);
+        last = (prev == total - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        for (int jj = threadIdx.x; jj <= k; jj += kDotThreads) {
+            double t = 0.0;
+            for (int b = 0; b < nbx; ++b)
+                t += __hip_atomic_load(&parts[(int64_t)jj * nbx + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            out[jj] = t;
+        }
+        if (threadIdx.x == 0) atomicExch(counter, 0u);
+    }
+}
+
+// w -= V[:, :k] h
+__global__ void orth_update_kernel(const double* __restrict__ V, int64_t ldv, int k, const double* __restrict__ h,
+                                   double* __restrict__ w, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = w[i];
+    for (int j = 0; j < k; ++j) s -= V[(int64_t)j * ldv + i] * h[j];
+    w[i] = s;
+}
+
+// v = w / sqrt(nrm2[idx])
+__global__ void scale_copy_kernel(const double* __restrict__ w, const double* __restrict__ nrm2, int idx, int n,
+                                  double* __restrict__ v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double inv = 1.0 / sqrt(nrm2[idx]);
+    v[i] = w[i] * inv;
+}
+
+// xc = x + V[:, :k] y
+__global__ void vy_plus_x_kernel(const double* __restrict__ V, int64_t ldv, int k, const double* __restrict__ y,
+                                 const double* __restrict__ x, int n, double* __restrict__ xc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += V[(int64_t)j * ldv + i] * y[j];
+    xc[i] = x[i] + s;
+}
+
+}  // namespace
+
+GemvPlan plan_gemv(int rows, int cols, int64_t ld, int elem_bytes, int num_cus)
+{
+    GemvPlan p;
+    p.rows = rows;
+    p.cols = cols;
+    p.ld = ld;
+    p.elem_bytes = elem_bytes;
+    const int rows_per_block = 64 * (16 / elem_bytes);
+    p.row_blocks = (int)((ld + rows_per_block - 1) / rows_per_block);
+    // enough workgroups for ~8 per CU, but keep >= 32 columns per wavefront
+    const int target = std::max(1, num_cus) * 8;
+    int splits = (target + p.row_blocks - 1) / p.row_blocks;
+    splits = std::min(splits, std::max(1, cols / 128));
+    p.splits = std::max(1, std::min(splits, 256));
+    return p;
+}
+
+size_t gemv_parts_elems(const GemvPlan& p)
+{
+    const int rows_per_block = 64 * (16 / p.elem_bytes);
+    return (size_t)p.splits * (size_t)p.row_blocks * rows_per_block;
+}
+
+hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double* parts, double* y, const double* b,
+                       int mode, hipStream_t s)
+{
+    const int rows_per_block = 64 * (16 / p.elem_bytes);
+    if (p.ld % rows_per_block != 0) return hipErrorInvalidValue;  // loads assume whole row blocks
+    const int64_t pstride = (int64_t)p.row_blocks * rows_per_block;
+    const int cps = (p.cols + p.splits - 1) / p.splits;
+    dim3 grid(p.row_blocks, p.splits);
+    if (p.elem_bytes == 8)
+        hipLaunchKernelGGL(gemv_parts_kernel<double>, grid, dim3(kGemvThreads), 0, s, p.cols, p.ld,
+                           static_cast<const double*>(A), x, parts, pstride, cps);
+    else
+        hipLaunchKernelGGL(gemv_parts_kernel<float>, grid, dim3(kGemvThreads), 0, s, p.cols, p.ld,
+                           static_cast<const float*>(A), x, parts, pstride, cps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(reduce_splits_kernel, dim3((p.rows + 255) / 256), dim3(256), 0, s, parts, p.splits, pstride,
+                       p.rows, b, mode, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_splits(const double* parts, int splits, int64_t stride, int n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_splits_kernel, dim3((n + 255) / 256), dim3(256), 0, s, parts, splits, stride, n,
+                       nullptr, 0, out);
+    return hipGetLastError();
+}
+
+int multidot_blocks(int n)
+{
+    return std::max(1, std::min(64, (n + 4 * kDotThreads - 1) / (4 * kDotThreads)));
+}
+
+hipError_t launch_multidot(const double* V, int64_t ldv, int k, const double* w, int n, double* parts,
+                           unsigned int* counter, double* out, hipStream_t s)
+{
+    dim3 grid(multidot_blocks(n), k + 1);
+    hipLaunchKernelGGL(multidot_kernel, grid, dim3(kDotThreads), 0, s, V, ldv, k, w, n, parts, counter, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_orth_update(const double* V, int64_t ldv, int k, const double* h, double* w, int n,
+                              int fused_dots, double* parts, unsigned int* counter, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(orth_update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, V, ldv, k, h, w, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !fused_dots) return e;
+    return launch_multidot(V, ldv, k, w, n, parts, counter, out, s);
+}
+
+hipError_t launch_scale_copy(const double* w, const double* nrm2, int idx, int n, double* v, hipStream_t s)
+{
+    hipLaunchKernelGGL(scale_copy_kernel, dim3((n + 255) / 256), dim3(256), 0, s, w, nrm2, idx, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_vy_plus_x(const double* V, int64_t ldv, int k, const double* y, const double* x, int n,
+                            double* xc, hipStream_t s)
+{
+    hipLaunchKernelGGL(vy_plus_x_kernel, dim3((n + 255) / 256), dim3(256), 0, s, V, ldv, k, y, x, n, xc);
+    return hipGetLastError();
+}
+
+}  // namespace rf

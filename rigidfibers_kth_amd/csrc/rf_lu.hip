@@ -1,0 +1,447 @@
+// rf_lu.hip -- dense LU with partial pivoting on gfx950 (the direct solver,
+// LAPACK dgesv at fortran/coeffs_and_vels.f90:62,158,333,514).
+//
+// Blocked right-looking factorisation, column-major, panel width NB:
+//   for each panel [k0, k1):
+//     for each column k of the panel: pivot search + row swap inside the
+//       panel + column scaling (one workgroup), rank-1 update of the rest of
+//       the panel (all CUs);
+//     apply the panel's row swaps to the other columns (laswp);
+//     U12 = L11^-1 A12 (unit lower triangular solve);
+//     A22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4f64 for FP64,
+//       v_mfma_f32_16x16x4f32 for the FP32 factorisation of the mixed mode).
+// Pivots: first index of the largest |a_ik| (LAPACK idamax), so the pivot
+// sequence is dgesv's whenever the arithmetic agrees.
+#include "rf_internal.h"
+#include "rf_lu.h"
+
+#include <algorithm>
+
+namespace rf {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kPivThreads = 1024;
+
+// Column k: pivot search over rows [k, n), swap rows k and p over columns
+// [c0, c1), scale the subdiagonal of column k.  One workgroup.
+template <typename T>
+__global__ void __launch_bounds__(kPivThreads) lu_pivot_kernel(T* A, int64_t lda, int n, int k, int c0, int c1,
+                                                               int* ipiv, int* info)
+{
+    __shared__ double sv[kPivThreads / 64];
+    __shared__ int si[kPivThreads / 64];
+    __shared__ int piv_s;
+    const T* col = A + (int64_t)k * lda;
+    double best = -1.0;
+    int bi = n;
+    for (int i = k + threadIdx.x; i < n; i += kPivThreads) {
+        const double v = fabs((double)col[i]);
+        if (v > best) {  // strict: keeps the first index among equals in this thread
+            best = v;
+            bi = i;
+        }
+    }
+    // wave reduction: max value, then smallest index
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ov > best || (ov == best && oi < bi)) {
+            best = ov;
+            bi = oi;
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sv[w] = best;
+        si[w] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = sv[0];
+        int idx = si[0];
+        for (int q = 1; q < kPivThreads / 64; ++q)
+            if (sv[q] > b || (sv[q] == b && si[q] < idx)) {
+                b = sv[q];
+                idx = si[q];
+            }
+        if (idx >= n) idx = k;
+        piv_s = idx;
+        ipiv[k] = idx;
+    }
+    __syncthreads();
+    const int p = piv_s;
+    if (p != k) {
+        for (int c = c0 + threadIdx.x; c < c1; c += kPivThreads) {
+            T* cc = A + (int64_t)c * lda;
+            const T t = cc[k];
+            cc[k] = cc[p];
+            cc[p] = t;
+        }
+    }
+    __syncthreads();
+    T* colw = A + (int64_t)k * lda;
+    const double akk = (double)colw[k];
+    if (akk == 0.0) {
+        if (threadIdx.x == 0 && *info == 0) *info = k + 1;
+        return;
+    }
+    const double r = 1.0 / akk;
+    for (int i = k + 1 + threadIdx.x; i < n; i += kPivThreads) colw[i] = (T)((double)colw[i] * r);
+}
+
+// A[i, c] -= A[i, k] * A[k, c] for i in (k, n), c in (k, c1)
+template <typename T>
+__global__ void lu_panel_update_kernel(T* A, int64_t lda, int n, int k, int c1)
+{
+    const int i = k + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double lik = (double)A[(int64_t)k * lda + i];
+    for (int c = k + 1; c < c1; ++c) {
+        T* cc = A + (int64_t)c * lda;
+        cc[i] = (T)((double)cc[i] - lik * (double)cc[k]);
+    }
+}
+
+// Row swaps ipiv[k0..k1) applied to columns outside [k0, k1) (LAPACK dlaswp).
+template <typename T>
+__global__ void lu_laswp_kernel(T* A, int64_t lda, int n, int k0, int k1, const int* __restrict__ ipiv)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int outside = n - (k1 - k0);
+    if (c >= outside) return;
+    if (c >= k0) c += (k1 - k0);
+    T* cc = A + (int64_t)c * lda;
+    for (int k = k0; k < k1; ++k) {
+        const int p = ipiv[k];
+        if (p != k) {
+            const T t = cc[k];
+            cc[k] = cc[p];
+            cc[p] = t;
+        }
+    }
+}
+
+// U12 = L11^-1 A12: columns c in [k1, n), rows [k0, k1).  One thread per
+// column; L11 staged in LDS.
+template <typename T, int NB>
+__global__ void __launch_bounds__(256) lu_trsm_kernel(T* A, int64_t lda, int n, int k0, int k1)
+{
+    __shared__ double L[NB][NB + 1];
+    const int nb = k1 - k0;
+    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+        const int r = e % nb, c = e / nb;
+        L[r][c] = (double)A[(int64_t)(k0 + c) * lda + k0 + r];
+    }
+    __syncthreads();
+    const int c = k1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    T* cc = A + (int64_t)c * lda + k0;
+    double x[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r) x[r] = (r < nb) ? (double)cc[r] : 0.0;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        const double xr = x[r];
+#pragma unroll
+        for (int q = r + 1; q < NB; ++q)
+            if (r < nb && q < nb) x[q] -= L[q][r] * xr;
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r)
+        if (r < nb) cc[r] = (T)x[r];
+}
+
+// ---- trailing update C -= L21 * U12 ------------------------------------
+// Workgroup tile 64x64, K staged 16 at a time in LDS.
+
+constexpr int kTile = 64;
+constexpr int kKc = 16;
+
+template <typename T>
+__device__ __forceinline__ void stage_tiles(const T* A, int64_t lda, int n, int rbase, int cbase, int k0, int kc,
+                                            int kcount, double (&As)[kTile][kKc + 1], double (&Bs)[kKc][kTile + 1])
+{
+    // As[r][kk] = L21[rbase + r, kc + kk] ; Bs[kk][c] = U12[kc + kk, cbase + c]
+    for (int e = threadIdx.x; e < kTile * kKc; e += 256) {
+        const int r = e % kTile, kk = e / kTile;
+        const int row = rbase + r, kcol = kc + kk;
+        As[r][kk] = (row < n && kk < kcount) ? (double)A[(int64_t)kcol * lda + row] : 0.0;
+    }
+    for (int e = threadIdx.x; e < kTile * kKc; e += 256) {
+        const int kk = e % kKc, c = e / kKc;
+        const int col = cbase + c, krow = kc + kk;
+        Bs[kk][c] = (col < n && kk < kcount) ? (double)A[(int64_t)col * lda + krow] : 0.0;
+    }
+}
+
+// VALU version (reference for the matrix-core kernels; any precision).
+template <typename T>
+__global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, int n, int k0, int k1)
+{
+    __shared__ double As[kTile][kKc + 1];
+    __shared__ double Bs[kKc][kTile + 1];
+    const int rbase = k1 + blockIdx.x * kTile, cbase = k1 + blockIdx.y * kTile;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    double acc[4][4] = {};
+    for (int kc = k0; kc < k1; kc += kKc) {
+        const int kcount = min(kKc, k1 - kc);
+        stage_tiles<T>(A, lda, n, rbase, cbase, k0, kc, kcount, As, Bs);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kKc; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = As[ty * 4 + u][kk];
+                b[u] = Bs[kk][tx * 4 + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = rbase + ty * 4 + u, col = cbase + tx * 4 + v;
+            if (row < n && col < n) {
+                T* p = A + (int64_t)col * lda + row;
+                *p = (T)((double)*p - acc[u][v]);
+            }
+        }
+}
+
+// Matrix-core version: each of the 4 wavefronts owns a 32x32 quarter of the
+// 64x64 tile = 2x2 blocks of 16x16.  For v_mfma_*_16x16x4 lane l holds
+// A[l%16][l/16], B[l/16][l%16] and D[4*(l/16)+r][l%16], r = 0..3.
+__global__ void __launch_bounds__(256) lu_gemm_mfma_f64_kernel(double* A, int64_t lda, int n, int k0, int k1)
+{
+    __shared__ double As[kTile][kKc + 1];
+    __shared__ double Bs[kKc][kTile + 1];
+    const int rbase = k1 + blockIdx.x * kTile, cbase = k1 + blockIdx.y * kTile;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    d4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int kc = k0; kc < k1; kc += kKc) {
+        const int kcount = min(kKc, k1 - kc);
+        stage_tiles<double>(A, lda, n, rbase, cbase, k0, kc, kcount, As, Bs);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kKc; ks += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[wr + i * 16 + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[ks + (lane >> 4)][wc + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rbase + wr + i * 16 + 4 * (lane >> 4) + r;
+                const int col = cbase + wc + j * 16 + (lane & 15);
+                if (row < n && col < n) {
+                    double* p = A + (int64_t)col * lda + row;
+                    *p -= acc[i][j][r];
+                }
+            }
+}
+
+__global__ void __launch_bounds__(256) lu_gemm_mfma_f32_kernel(float* A, int64_t lda, int n, int k0, int k1)
+{
+    __shared__ double As[kTile][kKc + 1];
+    __shared__ double Bs[kKc][kTile + 1];
+    const int rbase = k1 + blockIdx.x * kTile, cbase = k1 + blockIdx.y * kTile;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    f4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int kc = k0; kc < k1; kc += kKc) {
+        const int kcount = min(kKc, k1 - kc);
+        stage_tiles<float>(A, lda, n, rbase, cbase, k0, kc, kcount, As, Bs);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kKc; ks += 4) {
+            float a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = (float)As[wr + i * 16 + (lane & 15)][ks + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = (float)Bs[ks + (lane >> 4)][wc + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = rbase + wr + i * 16 + 4 * (lane >> 4) + r;
+                const int col = cbase + wc + j * 16 + (lane & 15);
+                if (row < n && col < n) {
+                    float* p = A + (int64_t)col * lda + row;
+                    *p -= acc[i][j][r];
+                }
+            }
+}
+
+// ---- solves with the factors (FP64 vectors) ------------------------------
+
+__global__ void apply_pivots_kernel(double* b, int n, const int* __restrict__ ipiv)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int k = 0; k < n; ++k) {
+        const int p = ipiv[k];
+        if (p != k) {
+            const double t = b[k];
+            b[k] = b[p];
+            b[p] = t;
+        }
+    }
+}
+
+// Solve the diagonal block [k0,k1) of L (unit lower, lower=1) or U
+// (upper, lower=0) for the vector segment in place.  One workgroup of 64.
+template <typename T>
+__global__ void trsv_diag_kernel(const T* A, int64_t lda, double* x, int k0, int k1, int lower)
+{
+    __shared__ double xs[64];
+    const int nb = k1 - k0;
+    const int t = threadIdx.x;
+    if (t < nb) xs[t] = x[k0 + t];
+    __syncthreads();
+    if (lower) {
+        for (int r = 0; r < nb; ++r) {
+            const double xr = xs[r];
+            if (t > r && t < nb) xs[t] -= (double)A[(int64_t)(k0 + r) * lda + k0 + t] * xr;
+            __syncthreads();
+        }
+    } else {
+        for (int r = nb - 1; r >= 0; --r) {
+            if (t == r) xs[r] = xs[r] / (double)A[(int64_t)(k0 + r) * lda + k0 + r];
+            __syncthreads();
+            const double xr = xs[r];
+            if (t < r) xs[t] -= (double)A[(int64_t)(k0 + r) * lda + k0 + t] * xr;
+            __syncthreads();
+        }
+    }
+    if (t < nb) x[k0 + t] = xs[t];
+}
+
+// x[i] -= sum_{j in [k0,k1)} A[i, j] x[j] for rows i in [r0, r1)
+template <typename T>
+__global__ void trsv_update_kernel(const T* A, int64_t lda, double* x, int k0, int k1, int r0, int r1)
+{
+    const int i = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r1) return;
+    double s = 0.0;
+    for (int j = k0; j < k1; ++j) s += (double)A[(int64_t)j * lda + i] * x[j];
+    x[i] -= s;
+}
+
+template <typename T>
+hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, hipStream_t s)
+{
+    for (int k0 = 0; k0 < n; k0 += nb) {
+        const int k1 = std::min(n, k0 + nb);
+        for (int k = k0; k < k1; ++k) {
+            hipLaunchKernelGGL(lu_pivot_kernel<T>, dim3(1), dim3(kPivThreads), 0, s, A, lda, n, k, k0, k1, ipiv, info);
+            if (k + 1 < k1) {
+                const int rows = n - k - 1;
+                if (rows > 0)
+                    hipLaunchKernelGGL(lu_panel_update_kernel<T>, dim3((rows + 255) / 256), dim3(256), 0, s, A, lda,
+                                       n, k, k1);
+            }
+        }
+        const int outside = n - (k1 - k0);
+        if (outside > 0)
+            hipLaunchKernelGGL(lu_laswp_kernel<T>, dim3((outside + 255) / 256), dim3(256), 0, s, A, lda, n, k0, k1,
+                               ipiv);
+        if (k1 < n) {
+            hipLaunchKernelGGL((lu_trsm_kernel<T, 64>), dim3((n - k1 + 255) / 256), dim3(256), 0, s, A, lda, n, k0,
+                               k1);
+            const int tiles = (n - k1 + kTile - 1) / kTile;
+            dim3 grid(tiles, tiles);
+            if (use_mfma) {
+                if (sizeof(T) == 8)
+                    hipLaunchKernelGGL(lu_gemm_mfma_f64_kernel, grid, dim3(256), 0, s, reinterpret_cast<double*>(A),
+                                       lda, n, k0, k1);
+                else
+                    hipLaunchKernelGGL(lu_gemm_mfma_f32_kernel, grid, dim3(256), 0, s, reinterpret_cast<float*>(A),
+                                       lda, n, k0, k1);
+            } else {
+                hipLaunchKernelGGL(lu_gemm_valu_kernel<T>, grid, dim3(256), 0, s, A, lda, n, k0, k1);
+            }
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T>
+hipError_t solve_impl(const T* LU, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s)
+{
+    constexpr int B = 64;
+    hipLaunchKernelGGL(apply_pivots_kernel, dim3(1), dim3(64), 0, s, x, n, ipiv);
+    for (int k0 = 0; k0 < n; k0 += B) {
+        const int k1 = std::min(n, k0 + B);
+        hipLaunchKernelGGL(trsv_diag_kernel<T>, dim3(1), dim3(64), 0, s, LU, lda, x, k0, k1, 1);
+        if (k1 < n)
+            hipLaunchKernelGGL(trsv_update_kernel<T>, dim3((n - k1 + 255) / 256), dim3(256), 0, s, LU, lda, x, k0, k1,
+                               k1, n);
+    }
+    const int last0 = ((n - 1) / B) * B;
+    for (int k0 = last0; k0 >= 0; k0 -= B) {
+        const int k1 = std::min(n, k0 + B);
+        hipLaunchKernelGGL(trsv_diag_kernel<T>, dim3(1), dim3(64), 0, s, LU, lda, x, k0, k1, 0);
+        if (k0 > 0)
+            hipLaunchKernelGGL(trsv_update_kernel<T>, dim3((k0 + 255) / 256), dim3(256), 0, s, LU, lda, x, k0, k1, 0,
+                               k0);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
+                     hipStream_t s)
+{
+    if (nb < 1 || nb > 64) return hipErrorInvalidValue;
+    if (elem_bytes == 8) return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, s);
+    return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, s);
+}
+
+hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s)
+{
+    if (elem_bytes == 8) return solve_impl<double>(static_cast<const double*>(LU), lda, n, ipiv, x, s);
+    return solve_impl<float>(static_cast<const float*>(LU), lda, n, ipiv, x, s);
+}
+
+}  // namespace rf

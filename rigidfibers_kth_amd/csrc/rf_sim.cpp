@@ -1,0 +1,871 @@
+// rf_sim.cpp -- the rf_sim handle: device memory, the timestep phases and the
+// solvers, behind the C ABI of include/rigidfibers.h.
+//
+// One handle = one HIP stream on the current device.  All state lives in HBM
+// for the whole run; the host sees O(restart) scalars per GMRES iteration
+// (the Hessenberg column) and nothing else inside a step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rf_internal.h"
+#include "rf_lu.h"
+#include "rf_sim.h"
+
+namespace rf {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+
+int assemble_targets_per_block(int LQ);
+
+// fortran/quadrature.f90:6-96, fortran/matrix_assembly.f90:8-57,294-305,370
+void build_tables(int M, int noqi, double eps, Tables* t)
+{
+    t->M = M;
+    t->LQ = 3 * noqi;
+    t->eps = eps;
+    const double c = std::log(eps * eps * std::exp(1.0));
+    t->d = -c;
+    t->e = 2.0;
+    t->D1 = 0.75 / (t->d - 2.0);
+    double lambda = 2.0;
+    for (int k = 1; k <= M; ++k) {
+        if (k > 1) lambda += 2.0 / k;
+        t->Ek[k] = (t->d - t->e - lambda) / 2.0 / (t->d - lambda);
+        t->gammak[k] = 0.5 * (2.0 * k + 1.0) / (t->d + t->e - lambda);
+    }
+    const double p0[3] = {-std::sqrt(15.0) / 5.0, 0.0, std::sqrt(15.0) / 5.0};
+    const double w0[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    double a = -1.0;
+    const double iv = 2.0 / noqi;
+    for (int i = 0; i < noqi; ++i) {
+        for (int q = 0; q < 3; ++q) {
+            t->pv[3 * i + q] = (2.0 * a + iv + p0[q] * iv) / 2.0;
+            t->wv[3 * i + q] = w0[q] / noqi;
+        }
+        a += iv;
+    }
+    for (int p = 0; p < t->LQ; ++p) {
+        // Legendre three-term recurrence for P_0..P_M at the point
+        const double x = t->pv[p];
+        double Pkm1 = 1.0, Pk = x;
+        t->wp[0 * t->LQ + p] = t->wv[p];
+        if (M >= 1) t->wp[1 * t->LQ + p] = t->wv[p] * x;
+        for (int k = 1; k < M; ++k) {
+            const double Pk1 = ((2.0 * k + 1.0) * x * Pk - k * Pkm1) / (k + 1.0);
+            Pkm1 = Pk;
+            Pk = Pk1;
+            t->wp[(k + 1) * t->LQ + p] = t->wv[p] * Pk;
+        }
+    }
+}
+
+}  // namespace rf
+
+using namespace rf;
+
+#define HIP_TRY(call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            rf::set_error(std::string(#call) + ": " + hipGetErrorString(e_));                   \
+            return RF_EHIP;                                                                       \
+        }                                                                                         \
+    } while (0)
+
+namespace {
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+template <typename T>
+int dev_alloc(T** p, size_t elems)
+{
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(elems, 1) * sizeof(T));
+    if (e != hipSuccess) {
+        rf::set_error(std::string("hipMalloc(") + std::to_string(elems * sizeof(T)) + " bytes): " +
+                      hipGetErrorString(e));
+        return RF_ENOMEM;
+    }
+    *p = static_cast<T*>(q);
+    return RF_OK;
+}
+
+struct PhaseTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace
+
+rf_sim::~rf_sim()
+{
+    for (void* q : allocs) (void)hipFree(q);
+    if (h_dots) (void)hipHostFree(h_dots);
+    if (h_y) (void)hipHostFree(h_y);
+    for (auto& e : events) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int rf_sim::alloc_all()
+{
+#define ALLOC(ptr, count)                                 \
+    do {                                                  \
+        int rc_ = dev_alloc(&(ptr), (count));             \
+        if (rc_) return rc_;                              \
+        allocs.push_back((void*)(ptr));                   \
+    } while (0)
+    const size_t N3 = 3 * (size_t)N;
+    for (int i = 0; i < 3; ++i) {
+        ALLOC(d_pos[i], N3);
+        ALLOC(d_orient[i], N3);
+    }
+    for (int i = 0; i < 2; ++i) {
+        ALLOC(d_vel[i], N3);
+        ALLOC(d_rot[i], N3);
+    }
+    ALLOC(d_force, N3);
+    ALLOC(d_wp, (size_t)(kMaxTerms + 1) * kMaxQuad);
+    ALLOC(d_pv, (size_t)kMaxQuad);
+    const size_t es = mixed ? sizeof(float) : sizeof(double);
+    {
+        void* q = nullptr;
+        if (hipMalloc(&q, (size_t)lda * n * es) != hipSuccess) {
+            rf::set_error("hipMalloc of the system matrix (" + std::to_string((size_t)lda * n * es) +
+                          " bytes) failed");
+            return RF_ENOMEM;
+        }
+        allocs.push_back(q);
+        d_A = q;
+        if (hipMalloc(&q, (size_t)ldw * wcols * es) != hipSuccess) {
+            rf::set_error("hipMalloc of the velocity operator failed");
+            return RF_ENOMEM;
+        }
+        allocs.push_back(q);
+        d_W = q;
+    }
+    ALLOC(d_b, (size_t)lda);
+    ALLOC(d_bpart, (size_t)asm_splits * n);
+    ALLOC(d_x, (size_t)lda);
+    ALLOC(d_xprev, (size_t)lda);
+    ALLOC(d_guess, (size_t)lda);
+    ALLOC(d_fext, (size_t)wcols);
+    ALLOC(d_y6, (size_t)ldw);
+    const int m = restart;
+    ALLOC(d_V, (size_t)lda * (m + 1));
+    ALLOC(d_w, (size_t)lda);
+    ALLOC(d_r, (size_t)lda);
+    ALLOC(d_xc, (size_t)lda);
+    ALLOC(d_gparts, std::max(gemv_parts_elems(planA), gemv_parts_elems(planW)));
+    ALLOC(d_dparts, (size_t)multidot_blocks(n) * (m + 2) + 64);
+    ALLOC(d_dots, (size_t)(3 * (m + 2) + 8));
+    ALLOC(d_y, (size_t)(m + 2));
+    ALLOC(d_counter, 1);
+#undef ALLOC
+    HIP_TRY(hipMemsetAsync(d_counter, 0, sizeof(unsigned int), stream));
+    // zero whole buffers once so the padding rows of A and W read as 0
+    HIP_TRY(hipMemsetAsync(d_A, 0, (size_t)lda * n * es, stream));
+    HIP_TRY(hipMemsetAsync(d_W, 0, (size_t)ldw * wcols * es, stream));
+    HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * lda, stream));
+    HIP_TRY(hipMemsetAsync(d_xprev, 0, sizeof(double) * lda, stream));
+    HIP_TRY(hipMemsetAsync(d_b, 0, sizeof(double) * lda, stream));
+    HIP_TRY(hipMemsetAsync(d_V, 0, sizeof(double) * lda * (m + 1), stream));
+    HIP_TRY(hipMemsetAsync(d_w, 0, sizeof(double) * lda, stream));
+    HIP_TRY(hipMemsetAsync(d_r, 0, sizeof(double) * lda, stream));
+    HIP_TRY(hipMemsetAsync(d_xc, 0, sizeof(double) * lda, stream));
+    return RF_OK;
+}
+
+double rf_sim::event_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventSynchronize(b) != hipSuccess) return 0.0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return (double)ms;
+}
+
+int rf_sim::gemv_A(const double* x, double* y, const double* b, int mode)
+{
+    if (p.profile) HIP_TRY(hipEventRecord(ev_k0, stream));
+    HIP_TRY(launch_gemv(planA, d_A, x, d_gparts, y, b, mode, stream));
+    if (p.profile) {
+        HIP_TRY(hipEventRecord(ev_k1, stream));
+        stats.gemv_ms_total += event_ms(ev_k0, ev_k1);
+        stats.gemv_launches += 1;
+    }
+    return RF_OK;
+}
+
+int rf_sim::fetch_dots(int count)
+{
+    HIP_TRY(hipMemcpyAsync(h_dots, d_dots, sizeof(double) * count, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return RF_OK;
+}
+
+// ||v||^2 into h_dots[0]
+int rf_sim::norm2(const double* v, double* out)
+{
+    HIP_TRY(launch_multidot(nullptr, 0, 0, v, n, d_dparts, d_counter, d_dots, stream));
+    int rc = fetch_dots(1);
+    if (rc) return rc;
+    *out = h_dots[0];
+    return RF_OK;
+}
+
+// Upper-triangular solve H(0:k,0:k) y = g(0:k) (dtrsv 'U','N','N'), host.
+static void hess_solve(const std::vector<double>& H, int ldh, const std::vector<double>& g, int k,
+                       std::vector<double>& y)
+{
+    for (int i = 0; i < k; ++i) y[i] = g[i];
+    for (int i = k - 1; i >= 0; --i) {
+        y[i] = y[i] / H[i + (size_t)i * ldh];
+        for (int r = 0; r < i; ++r) y[r] -= H[r + (size_t)i * ldh] * y[i];
+    }
+}
+
+// drotg / drot of fortran/blas_rot.f:78-148
+static void givens(double da, double db, double* c, double* s)
+{
+    const double roe = (std::fabs(da) > std::fabs(db)) ? da : db;
+    const double scale = std::fabs(da) + std::fabs(db);
+    if (scale == 0.0) {
+        *c = 1.0;
+        *s = 0.0;
+        return;
+    }
+    double r = scale * std::sqrt((da / scale) * (da / scale) + (db / scale) * (db / scale));
+    r = std::copysign(1.0, roe) * r;
+    *c = da / r;
+    *s = db / r;
+}
+static void rot1(double* x, double* y, double c, double s)
+{
+    const double t = c * (*x) + s * (*y);
+    *y = c * (*y) - s * (*x);
+    *x = t;
+}
+
+// Restarted GMRES with the CERFACS dGMRES semantics the reference uses
+// (fortran/coeffs_and_vels.f90:253-339, fortran/dGMRES.f:736-1495):
+// no preconditioner, stop when the Arnoldi backward error |g_{j+1}|/||b||
+// and then the true one ||b - A x||/||b|| are <= tol, true residual at each
+// restart, d_x holds the initial guess.  Orthogonalisation is classical
+// Gram-Schmidt applied twice (one fused multi-dot per pass instead of j
+// sequential dots); in exact arithmetic it builds the same Krylov basis and
+// the same iterates as the reference's modified Gram-Schmidt.
+int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
+{
+    const int m = restart;
+    const int maxit = p.gmres_max_iters;
+    const double tol = (mixed && p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : p.gmres_tol;
+    const int ldh = m + 1;
+    std::vector<double> H((size_t)(m + 1) * (m + 2), 0.0), g(m + 1, 0.0), cs(m + 1), sn(m + 1), y(m + 1);
+    int rc;
+    double bn2;
+    if ((rc = norm2(d_b, &bn2))) return rc;
+    const double bn = std::sqrt(bn2);
+    *iters_out = 0;
+    *status_out = 0;
+    if (bn == 0.0) {
+        HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, stream));
+        *be_out = 0.0;
+        return RF_OK;
+    }
+    // r0 = b - A x
+    if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
+    double beta2;
+    if ((rc = norm2(d_r, &beta2))) return rc;
+    double beta = std::sqrt(beta2);
+    if (beta == 0.0) {
+        *be_out = 0.0;
+        return RF_OK;
+    }
+    // norm2 left ||r0||^2 in d_dots[0]
+    HIP_TRY(launch_scale_copy(d_r, d_dots, 0, n, d_V, stream));
+    int iterOut = 0;
+    const int64_t ldv = lda;
+    double* D1 = d_dots;
+    double* D2 = d_dots + (m + 2);
+    double* D3 = d_dots + 2 * (m + 2);
+    for (;;) {
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        double bea = 0.0;
+        bool have_xc = false;
+        for (int jH = 0; jH < m; ++jH) {
+            const int k = jH + 1;
+            double* vj = d_V + (size_t)jH * ldv;
+            if ((rc = gemv_A(vj, d_w, nullptr, 0))) return rc;
+            HIP_TRY(launch_multidot(d_V, ldv, k, d_w, n, d_dparts, d_counter, D1, stream));
+            HIP_TRY(launch_orth_update(d_V, ldv, k, D1, d_w, n, 1, d_dparts, d_counter, D2, stream));
+            HIP_TRY(launch_orth_update(d_V, ldv, k, D2, d_w, n, 0, nullptr, nullptr, nullptr, stream));
+            HIP_TRY(launch_multidot(nullptr, 0, 0, d_w, n, d_dparts, d_counter, D3, stream));
+            if (k < m) HIP_TRY(launch_scale_copy(d_w, D3, 0, n, d_V + (size_t)k * ldv, stream));
+            if ((rc = fetch_dots(3 * (m + 2)))) return rc;
+            for (int j = 0; j < k; ++j) H[j + (size_t)jH * ldh] = h_dots[j] + h_dots[(m + 2) + j];
+            const double dnormw = std::sqrt(h_dots[2 * (m + 2)]);
+            H[k + (size_t)jH * ldh] = dnormw;
+            for (int j = 0; j < jH; ++j) rot1(&H[j + (size_t)jH * ldh], &H[j + 1 + (size_t)jH * ldh], cs[j], sn[j]);
+            givens(H[jH + (size_t)jH * ldh], H[k + (size_t)jH * ldh], &cs[jH], &sn[jH]);
+            rot1(&g[jH], &g[k], cs[jH], sn[jH]);
+            rot1(&H[jH + (size_t)jH * ldh], &H[k + (size_t)jH * ldh], cs[jH], sn[jH]);
+            H[k + (size_t)jH * ldh] = 0.0;
+            bea = std::fabs(g[k]) / bn;
+            const int total = iterOut * m + k;
+            last_bea = bea;
+            if (bea <= tol || total >= maxit) {
+                hess_solve(H, ldh, g, k, y);
+                std::copy(y.begin(), y.begin() + k, h_y);
+                HIP_TRY(hipMemcpyAsync(d_y, h_y, sizeof(double) * k, hipMemcpyHostToDevice, stream));
+                HIP_TRY(launch_vy_plus_x(d_V, ldv, k, d_y, d_x, n, d_xc, stream));
+                have_xc = true;
+                if ((rc = gemv_A(d_xc, d_r, d_b, 1))) return rc;
+                double rr;
+                if ((rc = norm2(d_r, &rr))) return rc;
+                const double be = std::sqrt(rr) / bn;
+                if (be <= tol || total >= maxit) {
+                    HIP_TRY(hipMemcpyAsync(d_x, d_xc, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+                    *iters_out = total;
+                    *be_out = be;
+                    *status_out = (be <= tol) ? 0 : -4;
+                    return RF_OK;
+                }
+            }
+        }
+        ++iterOut;
+        if (bea > tol || !have_xc) {
+            hess_solve(H, ldh, g, m, y);
+            std::copy(y.begin(), y.begin() + m, h_y);
+            HIP_TRY(hipMemcpyAsync(d_y, h_y, sizeof(double) * m, hipMemcpyHostToDevice, stream));
+            HIP_TRY(launch_vy_plus_x(d_V, ldv, m, d_y, d_x, n, d_xc, stream));
+        }
+        HIP_TRY(hipMemcpyAsync(d_x, d_xc, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
+        if ((rc = norm2(d_r, &beta2))) return rc;
+        beta = std::sqrt(beta2);
+        if (beta == 0.0) {
+            *iters_out = iterOut * m;
+            *be_out = 0.0;
+            return RF_OK;
+        }
+        HIP_TRY(launch_scale_copy(d_r, d_dots, 0, n, d_V, stream));
+    }
+}
+
+// Direct solve (dgesv, fortran/coeffs_and_vels.f90:62,158): LU with partial
+// pivoting of a copy of A, then the two triangular solves.  In the mixed mode
+// the FP32 matrix is factorised in FP32 (matrix-core trailing updates) and the
+// solution is refined against the FP64-arithmetic residual b - A x.
+int rf_sim::solve_lu(double* be_out)
+{
+    const int es = mixed ? 4 : 8;
+    if (!d_LU) {
+        void* q = nullptr;
+        if (hipMalloc(&q, (size_t)lda * n * es) != hipSuccess) {
+            rf::set_error("solve_lu: cannot allocate the LU workspace");
+            return RF_ENOMEM;
+        }
+        allocs.push_back(q);
+        d_LU = q;
+        if (hipMalloc(&q, sizeof(int) * (n + 1)) != hipSuccess) {
+            rf::set_error("solve_lu: cannot allocate pivots");
+            return RF_ENOMEM;
+        }
+        allocs.push_back(q);
+        d_ipiv = static_cast<int*>(q);
+        d_info = d_ipiv + n;
+    }
+    if (p.profile) HIP_TRY(hipEventRecord(ev_k0, stream));
+    HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(hipMemsetAsync(d_info, 0, sizeof(int), stream));
+    const int nb = (p.lu_block > 0) ? std::min(64, p.lu_block) : 64;
+    HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, 1, stream));
+    if (p.profile) {
+        HIP_TRY(hipEventRecord(ev_k1, stream));
+        stats.lu_ms_total += event_ms(ev_k0, ev_k1);
+    }
+    stats.lu_factorizations += 1;
+    int info = 0;
+    HIP_TRY(hipMemcpyAsync(h_dots, d_info, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    std::memcpy(&info, h_dots, sizeof(int));
+    if (info != 0) {
+        rf::set_error("solve_lu: matrix is singular (U(" + std::to_string(info) + "," + std::to_string(info) +
+                      ") = 0)");
+        return RF_ENOCONV;
+    }
+    int rc;
+    double bn2;
+    if ((rc = norm2(d_b, &bn2))) return rc;
+    const double bn = std::sqrt(bn2);
+    if (!mixed) {
+        HIP_TRY(hipMemcpyAsync(d_x, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(lu_solve(d_LU, es, lda, n, d_ipiv, d_x, stream));
+        if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
+        double rr;
+        if ((rc = norm2(d_r, &rr))) return rc;
+        *be_out = (bn > 0.0) ? std::sqrt(rr) / bn : 0.0;
+        return RF_OK;
+    }
+    // iterative refinement: x += LU^-1 (b - A x)
+    const double tol = (p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : p.gmres_tol;
+    HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, stream));
+    HIP_TRY(hipMemcpyAsync(d_r, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    double be = 1.0;
+    for (int it = 0; it < 30; ++it) {
+        HIP_TRY(lu_solve(d_LU, es, lda, n, d_ipiv, d_r, stream));
+        HIP_TRY(launch_axpby(n, 1.0, d_x, 1.0, d_r, d_x, stream));
+        if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
+        double rr;
+        if ((rc = norm2(d_r, &rr))) return rc;
+        be = (bn > 0.0) ? std::sqrt(rr) / bn : 0.0;
+        stats.last_gmres_iters = it + 1;
+        if (be <= tol) break;
+    }
+    *be_out = be;
+    return RF_OK;
+}
+
+int rf_sim::phase_begin()
+{
+    HIP_TRY(hipEventRecord(ev_p0, stream));
+    return RF_OK;
+}
+double rf_sim::phase_end()
+{
+    if (hipEventRecord(ev_p1, stream) != hipSuccess) return 0.0;
+    return event_ms(ev_p0, ev_p1);
+}
+
+int rf_sim::assemble()
+{
+    int rc = phase_begin();
+    if (rc) return rc;
+    AssembleArgs a;
+    a.N = N;
+    a.n = n;
+    a.lda = lda;
+    a.ldw = ldw;
+    a.analytic = p.use_analytical_integration;
+    a.rhs_z_only = 1;
+    a.splits = asm_splits;
+    a.pos = d_pos[lvl_three];
+    a.orient = d_orient[lvl_three];
+    a.force = d_force;
+    a.wp = d_wp;
+    a.pv = d_pv;
+    a.A = d_A;
+    a.W = d_W;
+    a.bpart = d_bpart;
+    a.mixed = mixed;
+    if (p.profile) HIP_TRY(hipEventRecord(ev_k0, stream));
+    HIP_TRY(launch_assemble(tab, a, stream));
+    if (p.profile) {
+        HIP_TRY(hipEventRecord(ev_k1, stream));
+        stats.assemble_ms_total += event_ms(ev_k0, ev_k1);
+        stats.assemble_launches += 1;
+    }
+    HIP_TRY(launch_reduce_splits(d_bpart, asm_splits, n, n, d_b, stream));
+    stats.ms_assemble = phase_end();
+    stats.total_ms_assemble += stats.ms_assemble;
+    assembled = true;
+    return RF_OK;
+}
+
+int rf_sim::solve()
+{
+    if (!assembled) {
+        rf::set_error("rf_solve: system not assembled");
+        return RF_EINVAL;
+    }
+    int rc = phase_begin();
+    if (rc) return rc;
+    // initial guess (fortran/advect_fibers.f90:464: init_guess = coeffvec)
+    if (p.warm_start == RF_WARM_ZERO) {
+        HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, stream));
+    } else if (p.warm_start == RF_WARM_EXTRAP && solves_done >= 2) {
+        HIP_TRY(launch_axpby(n, 2.0, d_x, -1.0, d_xprev, d_guess, stream));
+        HIP_TRY(hipMemcpyAsync(d_xprev, d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(d_x, d_guess, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(d_xprev, d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    }
+    int iters = 0, status = 0;
+    double be = 0.0;
+    if (p.use_direct_solver) {
+        rc = solve_lu(&be);
+        if (rc) return rc;
+        stats.last_solver = RF_SOLVER_LU;
+    } else {
+        rc = solve_gmres(&iters, &be, &status);
+        if (rc) return rc;
+        stats.last_solver = RF_SOLVER_GMRES;
+        if (status == -4) {  // fortran/coeffs_and_vels.f90:331-335
+            rc = solve_lu(&be);
+            if (rc) return rc;
+            stats.last_solver = RF_SOLVER_LU_FALLBACK;
+        }
+    }
+    stats.last_gmres_iters = iters;
+    stats.total_gmres_iters += iters;
+    stats.last_backward_error = be;
+    ++solves_done;
+    stats.ms_solve = phase_end();
+    stats.total_ms_solve += stats.ms_solve;
+    solved = true;
+    return RF_OK;
+}
+
+int rf_sim::update_velocities()
+{
+    if (!solved) {
+        rf::set_error("rf_update_velocities: no solution for this configuration");
+        return RF_EINVAL;
+    }
+    int rc = phase_begin();
+    if (rc) return rc;
+    // old=new; new=3-old (fortran/advect_fibers.f90:465-466)
+    vel_old = vel_new;
+    vel_new = 1 - vel_old;
+    HIP_TRY(launch_build_fext(N, M, d_force, d_x, d_fext, stream));
+    if (p.profile) HIP_TRY(hipEventRecord(ev_k0, stream));
+    HIP_TRY(launch_gemv(planW, d_W, d_fext, d_gparts, d_y6, nullptr, 0, stream));
+    if (p.profile) {
+        HIP_TRY(hipEventRecord(ev_k1, stream));
+        stats.velocity_gemv_ms_total += event_ms(ev_k0, ev_k1);
+        stats.velocity_gemv_launches += 1;
+    }
+    HIP_TRY(launch_velocity_epilogue(N, tab.d, tab.e, d_pos[lvl_three], d_orient[lvl_three], d_force, d_y6,
+                                     d_vel[vel_new], d_rot[vel_new], stream));
+    stats.ms_velocities = phase_end();
+    stats.total_ms_velocities += stats.ms_velocities;
+    have_velocities = true;
+    return RF_OK;
+}
+
+int rf_sim::advance()
+{
+    if (!have_velocities) {
+        rf::set_error("rf_advance: velocities not computed");
+        return RF_EINVAL;
+    }
+    int rc = phase_begin();
+    if (rc) return rc;
+    // one=two; two=three; three=6-(one+two)  (fortran/advect_fibers.f90:479-481)
+    lvl_one = lvl_two;
+    lvl_two = lvl_three;
+    lvl_three = 3 - (lvl_one + lvl_two);
+    const int first = (steps_taken == 0) ? 1 : 0;
+    HIP_TRY(launch_advance(N, first, p.timestep, d_pos[lvl_one], d_pos[lvl_two], d_orient[lvl_one],
+                           d_orient[lvl_two], d_vel[vel_old], d_vel[vel_new], d_rot[vel_old], d_rot[vel_new],
+                           d_pos[lvl_three], d_orient[lvl_three], stream));
+    stats.ms_advance = phase_end();
+    stats.total_ms_advance += stats.ms_advance;
+    ++steps_taken;
+    stats.steps_taken = steps_taken;
+    assembled = solved = have_velocities = false;
+    return RF_OK;
+}
+
+int rf_sim::step()
+{
+    int rc;
+    if ((rc = assemble())) return rc;
+    if ((rc = solve())) return rc;
+    if ((rc = update_velocities())) return rc;
+    return advance();
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" const char* rf_last_error(void) { return g_error.c_str(); }
+extern "C" const char* rf_version(void) { return "rigidfibers-mi355x 0.1 (gfx950)"; }
+
+extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const double* pos, const double* orient,
+                         rf_sim** out)
+{
+    if (!params || !pos || !orient || !out) {
+        set_error("rf_create: null argument");
+        return RF_EINVAL;
+    }
+    *out = nullptr;
+    const rf_params& P = *params;
+    if (num_fibers < 1 || P.num_terms < 1 || P.num_terms > kMaxTerms || P.num_quad_intervals < 1 ||
+        3 * P.num_quad_intervals > kMaxQuad || !(P.slenderness > 0.0) || P.gmres_restart < 1 ||
+        P.gmres_restart > 60 || P.gmres_max_iters < 1 || (P.precision != RF_PREC_FP64 && P.precision != RF_PREC_MIXED)) {
+        set_error("rf_create: invalid parameters (need N>=1, 1<=M<=8, 1<=NoQI<=32, eps>0, 1<=restart<=60)");
+        return RF_EINVAL;
+    }
+    int dev = 0, ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) {
+        set_error(std::string("rf_create: no HIP device: ") + hipGetErrorString(e));
+        return RF_EHIP;
+    }
+    rf_sim* s = new rf_sim();
+    s->p = P;
+    s->N = num_fibers;
+    s->M = P.num_terms;
+    s->n = 3 * s->M * s->N;
+    s->mixed = (P.precision == RF_PREC_MIXED) ? 1 : 0;
+    s->restart = P.gmres_restart;
+    build_tables(s->M, P.num_quad_intervals, P.slenderness, &s->tab);
+    s->LQ = s->tab.LQ;
+    s->lda = round_up(s->n, 256);
+    s->ldw = round_up(6 * (int64_t)s->N, 256);
+    s->wcols = 3 * (s->M + 1) * s->N;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&s->num_cus, hipDeviceAttributeMultiprocessorCount,
+                                                                  dev) != hipSuccess) {
+        delete s;
+        set_error("rf_create: cannot query the device");
+        return RF_EHIP;
+    }
+    const int T = assemble_targets_per_block(s->LQ);
+    const int tiles = (s->N + T - 1) / T;
+    s->asm_splits = std::max(1, std::min(std::min(64, s->N), (4096 + tiles - 1) / tiles));
+    const int es = s->mixed ? 4 : 8;
+    s->planA = plan_gemv(s->n, s->n, s->lda, es, s->num_cus);
+    s->planW = plan_gemv(6 * s->N, s->wcols, s->ldw, es, s->num_cus);
+    if (hipHostMalloc((void**)&s->h_dots, sizeof(double) * (3 * (s->restart + 2) + 8)) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_y, sizeof(double) * (s->restart + 2)) != hipSuccess) {
+        delete s;
+        set_error("rf_create: hipHostMalloc failed");
+        return RF_ENOMEM;
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        set_error("rf_create: hipStreamCreate failed");
+        return RF_EHIP;
+    }
+    for (hipEvent_t* ev : {&s->ev_p0, &s->ev_p1, &s->ev_k0, &s->ev_k1}) {
+        if (hipEventCreate(ev) != hipSuccess) {
+            delete s;
+            set_error("rf_create: hipEventCreate failed");
+            return RF_EHIP;
+        }
+        s->events.push_back(*ev);
+    }
+    int rc = s->alloc_all();
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    // state: current level = index 2 ("three"), orientations normalised
+    // (fortran/advect_fibers.f90:332-340)
+    std::vector<double> t(3 * (size_t)s->N), F(3 * (size_t)s->N);
+    for (int i = 0; i < s->N; ++i) {
+        const double* o = orient + 3 * i;
+        const double m = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+        if (!(m > 0.0)) {
+            delete s;
+            set_error("rf_create: zero orientation vector for fiber " + std::to_string(i));
+            return RF_EINVAL;
+        }
+        for (int c = 0; c < 3; ++c) {
+            t[3 * i + c] = o[c] / m;
+            F[3 * i + c] = P.external_force[c];
+        }
+    }
+    const size_t bytes3 = sizeof(double) * 3 * s->N;
+    e = hipMemcpyAsync(s->d_pos[s->lvl_three], pos, bytes3, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_orient[s->lvl_three], t.data(), bytes3, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_force, F.data(), bytes3, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_wp, s->tab.wp, sizeof(s->tab.wp), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_pv, s->tab.pv, sizeof(s->tab.pv), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        delete s;
+        set_error(std::string("rf_create: upload failed: ") + hipGetErrorString(e));
+        return RF_EHIP;
+    }
+    s->stats.matrix_elem_bytes = es;
+    s->stats.lda = (int32_t)s->lda;
+    s->stats.matrix_bytes = (double)s->lda * s->n * es;
+    *out = s;
+    return RF_OK;
+}
+
+extern "C" void rf_destroy(rf_sim* s)
+{
+    if (!s) return;
+    (void)hipStreamSynchronize(s->stream);
+    delete s;
+}
+
+#define CHECK_SIM(s)                          \
+    do {                                      \
+        if (!(s)) {                           \
+            set_error("null rf_sim handle"); \
+            return RF_EINVAL;                 \
+        }                                     \
+    } while (0)
+
+extern "C" int rf_step(rf_sim* s)
+{
+    CHECK_SIM(s);
+    return s->step();
+}
+extern "C" int rf_assemble(rf_sim* s)
+{
+    CHECK_SIM(s);
+    return s->assemble();
+}
+extern "C" int rf_solve(rf_sim* s)
+{
+    CHECK_SIM(s);
+    return s->solve();
+}
+extern "C" int rf_update_velocities(rf_sim* s)
+{
+    CHECK_SIM(s);
+    return s->update_velocities();
+}
+extern "C" int rf_advance(rf_sim* s)
+{
+    CHECK_SIM(s);
+    return s->advance();
+}
+
+static int copy_out(const rf_sim* s, double* dst, const double* src, size_t count)
+{
+    if (!dst) {
+        set_error("null output buffer");
+        return RF_EINVAL;
+    }
+    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return RF_OK;
+}
+
+extern "C" int rf_get_state(const rf_sim* s, double* pos, double* orient)
+{
+    CHECK_SIM(s);
+    int rc = copy_out(s, pos, s->d_pos[s->lvl_three], 3 * (size_t)s->N);
+    if (rc) return rc;
+    return copy_out(s, orient, s->d_orient[s->lvl_three], 3 * (size_t)s->N);
+}
+extern "C" int rf_get_velocities(const rf_sim* s, double* vel, double* rot)
+{
+    CHECK_SIM(s);
+    int rc = copy_out(s, vel, s->d_vel[s->vel_new], 3 * (size_t)s->N);
+    if (rc) return rc;
+    return copy_out(s, rot, s->d_rot[s->vel_new], 3 * (size_t)s->N);
+}
+extern "C" int rf_get_coefficients(const rf_sim* s, double* x)
+{
+    CHECK_SIM(s);
+    return copy_out(s, x, s->d_x, (size_t)s->n);
+}
+extern "C" int rf_get_rhs(const rf_sim* s, double* b)
+{
+    CHECK_SIM(s);
+    return copy_out(s, b, s->d_b, (size_t)s->n);
+}
+extern "C" int rf_set_coefficients(rf_sim* s, const double* x)
+{
+    CHECK_SIM(s);
+    if (!x) {
+        set_error("rf_set_coefficients: null");
+        return RF_EINVAL;
+    }
+    HIP_TRY(hipMemcpyAsync(s->d_x, x, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return RF_OK;
+}
+
+extern "C" int rf_get_matrix(const rf_sim* s, double* A, int64_t lda)
+{
+    CHECK_SIM(s);
+    if (!A || lda < s->n) {
+        set_error("rf_get_matrix: null buffer or lda < n");
+        return RF_EINVAL;
+    }
+    if (!s->mixed) {
+        HIP_TRY(hipMemcpy2DAsync(A, sizeof(double) * lda, s->d_A, sizeof(double) * s->lda, sizeof(double) * s->n,
+                                 s->n, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        return RF_OK;
+    }
+    std::vector<float> tmp((size_t)s->n * s->n);
+    HIP_TRY(hipMemcpy2DAsync(tmp.data(), sizeof(float) * s->n, s->d_A, sizeof(float) * s->lda, sizeof(float) * s->n,
+                             s->n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int c = 0; c < s->n; ++c)
+        for (int r = 0; r < s->n; ++r) A[(size_t)c * lda + r] = tmp[(size_t)c * s->n + r];
+    return RF_OK;
+}
+
+extern "C" int rf_get_stats(const rf_sim* s, rf_stats* st)
+{
+    CHECK_SIM(s);
+    if (!st) {
+        set_error("rf_get_stats: null");
+        return RF_EINVAL;
+    }
+    *st = s->stats;
+    return RF_OK;
+}
+
+extern "C" int rf_matvec(rf_sim* s, const double* x, double* y)
+{
+    CHECK_SIM(s);
+    if (!x || !y) {
+        set_error("rf_matvec: null");
+        return RF_EINVAL;
+    }
+    HIP_TRY(hipMemcpyAsync(s->d_xc, x, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    int rc = s->gemv_A(s->d_xc, s->d_r, nullptr, 0);
+    if (rc) return rc;
+    return copy_out(s, y, s->d_r, (size_t)s->n);
+}
+
+extern "C" int32_t rf_num_fibers(const rf_sim* s) { return s ? s->N : 0; }
+extern "C" int64_t rf_num_unknowns(const rf_sim* s) { return s ? s->n : 0; }
+
+static int write_vec(const std::string& path, const std::vector<double>& v, int fmt)
+{
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) {
+        set_error("cannot open " + path);
+        return RF_EIO;
+    }
+    const char* spec = (fmt == RF_FMT_F24_16) ? "%24.16f\n" : "%16.8f\n";
+    for (double x : v) std::fprintf(f, spec, x);
+    std::fclose(f);
+    return RF_OK;
+}
+
+// fortran/advect_fibers.f90:529-539, fortran/coeffs_and_vels.f90:55-72,633-642
+extern "C" int rf_write_step_outputs(const rf_sim* s, const char* dir, int32_t format)
+{
+    CHECK_SIM(s);
+    if (!dir) {
+        set_error("rf_write_step_outputs: null dir");
+        return RF_EINVAL;
+    }
+    const size_t N3 = 3 * (size_t)s->N;
+    std::vector<double> pos(N3), ori(N3), vel(N3), rot(N3), x(s->n), b(s->n);
+    int rc;
+    if ((rc = rf_get_state(s, pos.data(), ori.data()))) return rc;
+    if ((rc = rf_get_velocities(s, vel.data(), rot.data()))) return rc;
+    if ((rc = rf_get_coefficients(s, x.data()))) return rc;
+    if ((rc = rf_get_rhs(s, b.data()))) return rc;
+    const std::string d(dir);
+    if ((rc = write_vec(d + "/POS.out", pos, format))) return rc;
+    if ((rc = write_vec(d + "/ORIENT.out", ori, format))) return rc;
+    if ((rc = write_vec(d + "/TRANSVel.out", vel, format))) return rc;
+    if ((rc = write_vec(d + "/ROTVel.out", rot, format))) return rc;
+    if ((rc = write_vec(d + "/XVec.out", x, format))) return rc;
+    return write_vec(d + "/BVec.out", b, format);
+}

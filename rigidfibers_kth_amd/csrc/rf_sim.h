@@ -1,0 +1,84 @@
+// rf_sim.h -- definition of the opaque rf_sim handle (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "rf_internal.h"
+
+struct rf_sim {
+    rf_params p{};
+    int N = 0, M = 0, LQ = 0, n = 0;
+    int mixed = 0;
+    int restart = 10;
+    int num_cus = 256;
+    int64_t lda = 0, ldw = 0;
+    int wcols = 0;
+    int asm_splits = 1;
+    rf::Tables tab;
+    rf::GemvPlan planA, planW;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_p0 = nullptr, ev_p1 = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    std::vector<hipEvent_t> events;
+    std::vector<void*> allocs;
+
+    // time levels (fortran/advect_fibers.f90:294-298): lvl_three is current
+    int lvl_one = 0, lvl_two = 1, lvl_three = 2;
+    int vel_old = 0, vel_new = 1;
+    int64_t steps_taken = 0;
+    int64_t solves_done = 0;
+    bool assembled = false, solved = false, have_velocities = false;
+    double last_bea = 0.0;
+
+    double* d_pos[3] = {nullptr, nullptr, nullptr};
+    double* d_orient[3] = {nullptr, nullptr, nullptr};
+    double* d_vel[2] = {nullptr, nullptr};
+    double* d_rot[2] = {nullptr, nullptr};
+    double* d_force = nullptr;
+    double* d_wp = nullptr;
+    double* d_pv = nullptr;
+    void* d_A = nullptr;  // n x n, lda, double or float
+    void* d_W = nullptr;  // 6N x 3(M+1)N, ldw
+    double* d_b = nullptr;
+    double* d_bpart = nullptr;
+    double* d_x = nullptr;
+    double* d_xprev = nullptr;
+    double* d_guess = nullptr;
+    double* d_fext = nullptr;
+    double* d_y6 = nullptr;
+    // GMRES workspace
+    double* d_V = nullptr;
+    double* d_w = nullptr;
+    double* d_r = nullptr;
+    double* d_xc = nullptr;
+    double* d_gparts = nullptr;
+    double* d_dparts = nullptr;
+    double* d_dots = nullptr;
+    double* d_y = nullptr;
+    unsigned int* d_counter = nullptr;
+    double* h_dots = nullptr;  // pinned
+    double* h_y = nullptr;     // pinned
+    // LU workspace (allocated on first use)
+    void* d_LU = nullptr;
+    int* d_ipiv = nullptr;
+    int* d_info = nullptr;
+
+    rf_stats stats{};
+
+    ~rf_sim();
+    int alloc_all();
+    int assemble();
+    int solve();
+    int update_velocities();
+    int advance();
+    int step();
+    int gemv_A(const double* x, double* y, const double* b, int mode);
+    int fetch_dots(int count);
+    int norm2(const double* v, double* out);
+    int solve_gmres(int* iters, double* be, int* status);
+    int solve_lu(double* be);
+    int phase_begin();
+    double phase_end();
+    static double event_ms(hipEvent_t a, hipEvent_t b);
+};

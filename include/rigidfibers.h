@@ -84,7 +84,8 @@ typedef struct rf_params {
     int32_t profile;    /* 1: time every GEMV / assembly launch with HIP events */
     int32_t lu_block;   /* LU panel width (0 = default) */
     double mixed_inner_tol; /* mixed mode: tolerance of the FP32-operator solve (0 = default 1e-10) */
-    int32_t reserved[8];
+    int32_t lu_valu_update; /* 1: LU trailing update on the vector ALUs instead of the matrix cores */
+    int32_t reserved[7];
 } rf_params;
 
 typedef struct rf_stats {
@@ -158,12 +159,28 @@ int rf_get_stats(const rf_sim *sim, rf_stats *stats);
 /* Sets the coefficient vector (used as the next GMRES initial guess). */
 int rf_set_coefficients(rf_sim *sim, const double *x);
 
+/* Waits for all work queued on the handle's stream. */
+int rf_synchronize(rf_sim *sim);
+
 /* y = A x with the assembled device matrix (FP64 arithmetic), n doubles. */
 int rf_matvec(rf_sim *sim, const double *x, double *y);
 
 /* POS.out ORIENT.out TRANSVel.out ROTVel.out XVec.out BVec.out of the last
  * step into dir, RF_FMT_F16_8 or RF_FMT_F24_16. */
 int rf_write_step_outputs(const rf_sim *sim, const char *dir, int32_t format);
+
+/* Diagnostic with no counterpart in the reference: lane layout of the
+ * 16x16x4 matrix-core instructions used by the LU trailing update.  One
+ * wavefront runs v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 with
+ * operands that encode row (mode 0), column (mode 1) or the K slot (mode 2);
+ * out64 / out32 receive the 4 accumulator values of each of the 64 lanes
+ * (lane-major, 256 values each). */
+int rf_selftest_mfma_layout(int32_t mode, double *out64, float *out32);
+
+/* Diagnostic with no counterpart in the reference: streams `bytes` bytes
+ * through a plain device copy kernel `reps` times and returns the average
+ * kernel time in *ms (HBM calibration for the byte counters). */
+int rf_selftest_copy(int64_t bytes, int32_t reps, double *ms);
 
 /* Number of fibers / unknowns of a handle. */
 int32_t rf_num_fibers(const rf_sim *sim);

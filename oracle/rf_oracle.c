@@ -324,13 +324,19 @@ static void write_block(double *A, int n, int M, int LQ, const double *wv, const
                         const consts_t *K, int filno, int fno, int l,
                         const double *Gmat, const double ta[3])
 {
-    for (int kk = 1; kk <= M; ++kk) {
-        double th[6];
+    /* thvec(q) = sum(wv*Gmat(:,q)*LvecMat(:,kk)) for all kk at once; each
+     * entry still sums over p in increasing order. */
+    double thall[8][6];
+    for (int kk = 0; kk < M; ++kk)
+        for (int q = 0; q < 6; ++q) thall[kk][q] = 0.0;
+    for (int p = 0; p < LQ; ++p) {
         for (int q = 0; q < 6; ++q) {
-            double s = 0.0;
-            for (int p = 0; p < LQ; ++p) s += wv[p] * Gmat[6 * p + q] * lvec[(kk - 1) * LQ + p];
-            th[q] = s;
+            const double g = wv[p] * Gmat[6 * p + q];
+            for (int kk = 0; kk < M; ++kk) thall[kk][q] += g * lvec[kk * LQ + p];
         }
+    }
+    for (int kk = 1; kk <= M; ++kk) {
+        const double *th = thall[kk - 1];
         double Q1 = th[0] * ta[0] + th[3] * ta[1] + th[4] * ta[2];
         double Q2 = th[3] * ta[0] + th[1] * ta[1] + th[5] * ta[2];
         double Q3 = th[4] * ta[0] + th[5] * ta[1] + th[2] * ta[2];
@@ -357,21 +363,29 @@ void orc_assemble_matrix(int analytic, double eps, int N, int M,
                          int LQ, const double *pv, const double *wv,
                          const double *lvec, double *A)
 {
+    /* Each (filno, fno) block depends only on that pair, so the blocks are
+     * computed source-fiber (column-block) major here: a thread owns whole
+     * columns of the column-major matrix.  Values are those of the Fortran's
+     * filno-major loop (matrix_assembly.f90:499-626). */
     consts_t K;
     make_consts(eps, M, &K);
     int n = 3 * M * N;
-    memset(A, 0, sizeof(double) * (size_t)n * n);
-    for (int i = 0; i < n; ++i) A[(size_t)i + (size_t)i * n] = 1.0;
 #pragma omp parallel
     {
         double *Gall = (double *)malloc(sizeof(double) * LQ * 6 * (M + 1));
         double *Gmat = (double *)malloc(sizeof(double) * LQ * 6);
 #pragma omp for schedule(dynamic, 1)
-        for (int filno = 0; filno < N; ++filno) {
-            const double *xc = pos + 3 * filno, *ta = orient + 3 * filno;
-            for (int fno = 0; fno < N; ++fno) {
+        for (int fno = 0; fno < N; ++fno) {
+            /* zero this fiber's columns, identity on its diagonal block */
+            for (int c = 3 * M * fno; c < 3 * M * (fno + 1); ++c) {
+                double *col = A + (size_t)c * n;
+                memset(col, 0, sizeof(double) * n);
+                col[c] = 1.0;
+            }
+            const double *xcb = pos + 3 * fno, *tb = orient + 3 * fno;
+            for (int filno = 0; filno < N; ++filno) {
                 if (fno == filno) continue;
-                const double *xcb = pos + 3 * fno, *tb = orient + 3 * fno;
+                const double *xc = pos + 3 * filno, *ta = orient + 3 * filno;
                 if (analytic) {
                     for (int p = 0; p < LQ; ++p) {
                         double xbar[3] = {xc[0] + pv[p] * ta[0], xc[1] + pv[p] * ta[1],

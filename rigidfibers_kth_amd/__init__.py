@@ -51,7 +51,8 @@ class rf_params(ctypes.Structure):
         ("profile", ctypes.c_int32),
         ("lu_block", ctypes.c_int32),
         ("mixed_inner_tol", ctypes.c_double),
-        ("reserved", ctypes.c_int32 * 8),
+        ("lu_valu_update", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -122,7 +123,10 @@ _SIGNATURES = {
     "rf_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(rf_stats)]),
     "rf_set_coefficients": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "rf_matvec": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "rf_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "rf_selftest_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _dp]),
     "rf_write_step_outputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]),
+    "rf_selftest_mfma_layout": (ctypes.c_int, [ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_float)]),
     "rf_num_fibers": (ctypes.c_int32, [ctypes.c_void_p]),
     "rf_num_unknowns": (ctypes.c_int64, [ctypes.c_void_p]),
     "rf_last_error": (ctypes.c_char_p, []),
@@ -157,6 +161,22 @@ def _check(rc, where):
 
 def _p(a):
     return a.ctypes.data_as(_dp)
+
+
+def selftest_mfma_layout(mode):
+    """Accumulators (64 lanes x 4) of the probe wavefront; see include/rigidfibers.h."""
+    d = np.empty(256)
+    f = np.empty(256, dtype=np.float32)
+    _check(lib().rf_selftest_mfma_layout(mode, _p(d), f.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+           "rf_selftest_mfma_layout")
+    return d.reshape(64, 4), f.reshape(64, 4).astype(np.float64)
+
+
+def selftest_copy(nbytes, reps=10):
+    """Average time (ms) of a plain device copy kernel streaming nbytes."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().rf_selftest_copy(int(nbytes), int(reps), ctypes.byref(ms)), "rf_selftest_copy")
+    return ms.value
 
 
 def default_params(**kw):
@@ -297,6 +317,9 @@ class Simulation:
         y = np.empty(self.n)
         _check(lib().rf_matvec(self._h, _p(x), _p(y)), "rf_matvec")
         return y
+
+    def synchronize(self):
+        _check(lib().rf_synchronize(self._h), "rf_synchronize")
 
     def stats(self):
         s = rf_stats()

@@ -205,7 +205,7 @@ struct KArgs {
     double* bpart;
 };
 
-constexpr int kMaxItems = 4;  // phase-3 row items per thread
+constexpr int kMaxItems = 8;  // A-row items per thread (3M/LQ <= 8)
 
 template <int M, bool ANALYTIC, bool MIXED>
 __global__ void __launch_bounds__(256) assemble_kernel(const KArgs a)
@@ -386,20 +386,30 @@ hipError_t launch_m(const KArgs& k, dim3 grid, int threads, size_t lds, bool ana
 
 }  // namespace
 
-int assemble_targets_per_block(int LQ)
+size_t assemble_lds_bytes(int T, int LQ, int M)
 {
-    // T*LQ threads; prefer a whole number of 64-wide wavefronts <= 256.
+    const size_t NL = M + 1;
+    return sizeof(double) * ((size_t)T * NL * 6 * (LQ + 1) + (size_t)T * NL * NL * 6);
+}
+
+int assemble_targets_per_block(int LQ, int M)
+{
+    // T*LQ threads; prefer a whole number of 64-wide wavefronts <= 256 whose
+    // LDS fits in the 160 KiB of a CU.
+    const size_t cap = 160 * 1024;
     for (int thr = 256; thr >= 64; thr -= 64)
-        if (thr % LQ == 0) return thr / LQ;
-    return std::max(1, 256 / LQ);
+        if (thr % LQ == 0 && assemble_lds_bytes(thr / LQ, LQ, M) <= cap) return thr / LQ;
+    int T = std::max(1, 256 / LQ);
+    while (T > 1 && assemble_lds_bytes(T, LQ, M) > cap) --T;
+    return T;
 }
 
 hipError_t launch_assemble(const Tables& t, const AssembleArgs& a, hipStream_t s)
 {
     const int M = t.M, LQ = t.LQ;
-    const int T = a.targets_per_block > 0 ? a.targets_per_block : assemble_targets_per_block(LQ);
+    const int T = a.targets_per_block > 0 ? a.targets_per_block : assemble_targets_per_block(LQ, M);
     const int threads = T * LQ;
-    if (threads > 1024 || T * (3 * M + 6) > kMaxItems * threads) return hipErrorInvalidValue;
+    if (threads > 1024 || T * 3 * M > kMaxItems * threads) return hipErrorInvalidValue;
     KArgs k;
     k.N = a.N;
     k.n = a.n;
@@ -425,8 +435,7 @@ hipError_t launch_assemble(const Tables& t, const AssembleArgs& a, hipStream_t s
     k.A = a.A;
     k.W = a.W;
     k.bpart = a.bpart;
-    const int NL = M + 1;
-    const size_t lds = sizeof(double) * ((size_t)T * NL * 6 * (LQ + 1) + (size_t)T * NL * NL * 6);
+    const size_t lds = assemble_lds_bytes(T, LQ, M);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     dim3 grid((a.N + T - 1) / T, a.splits);
     const bool an = a.analytic != 0, mx = a.mixed != 0;

@@ -65,8 +65,11 @@ struct GemvPlan {
 GemvPlan plan_gemv(int rows, int cols, int64_t ld, int elem_bytes, int num_cus);
 size_t gemv_parts_elems(const GemvPlan& p);
 // mode: 0 -> y = A x ; 1 -> y = b - A x ; scale applied to A x first.
+// ev0/ev1 (optional) are recorded around the partial-product kernel only.
 hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double* parts, double* y,
-                       const double* b, int mode, hipStream_t s);
+                       const double* b, int mode, hipStream_t s, hipEvent_t ev0 = nullptr,
+                       hipEvent_t ev1 = nullptr);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // Deterministic reductions / vector kernels (rf_linalg.hip).
 hipError_t launch_reduce_splits(const double* parts, int splits, int64_t stride, int n, double* out,

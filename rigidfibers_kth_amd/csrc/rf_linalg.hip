@@ -201,7 +201,21 @@ __global__ void vy_plus_x_kernel(const double* __restrict__ V, int64_t ldv, int 
     xc[i] = x[i] + s;
 }
 
+__global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, size_t n4)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s)
+{
+    const size_t n4 = bytes / sizeof(double4);
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, static_cast<const double4*>(src),
+                       static_cast<double4*>(dst), n4);
+    return hipGetLastError();
+}
 
 GemvPlan plan_gemv(int rows, int cols, int64_t ld, int elem_bytes, int num_cus)
 {
@@ -227,13 +241,17 @@ size_t gemv_parts_elems(const GemvPlan& p)
 }
 
 hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double* parts, double* y, const double* b,
-                       int mode, hipStream_t s)
+                       int mode, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const int rows_per_block = 64 * (16 / p.elem_bytes);
     if (p.ld % rows_per_block != 0) return hipErrorInvalidValue;  // loads assume whole row blocks
     const int64_t pstride = (int64_t)p.row_blocks * rows_per_block;
     const int cps = (p.cols + p.splits - 1) / p.splits;
     dim3 grid(p.row_blocks, p.splits);
+    if (ev0) {
+        hipError_t e0 = hipEventRecord(ev0, s);
+        if (e0 != hipSuccess) return e0;
+    }
     if (p.elem_bytes == 8)
         hipLaunchKernelGGL(gemv_parts_kernel<double>, grid, dim3(kGemvThreads), 0, s, p.cols, p.ld,
                            static_cast<const double*>(A), x, parts, pstride, cps);
@@ -242,6 +260,10 @@ hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double
                            static_cast<const float*>(A), x, parts, pstride, cps);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (ev1) {
+        e = hipEventRecord(ev1, s);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(reduce_splits_kernel, dim3((p.rows + 255) / 256), dim3(256), 0, s, parts, p.splits, pstride,
                        p.rows, b, mode, y);
     return hipGetLastError();

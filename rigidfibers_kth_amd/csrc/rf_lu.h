@@ -15,4 +15,7 @@ hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* i
 // x <- U^-1 L^-1 P x with FP64 vectors.
 hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s);
 
+// Lane-layout probe of the 16x16x4 matrix-core instructions (one wavefront).
+hipError_t mfma_probe(int mode, double* out64, float* out32, hipStream_t s);
+
 }  // namespace rf

@@ -220,7 +220,9 @@ __global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, in
 
 // Matrix-core version: each of the 4 wavefronts owns a 32x32 quarter of the
 // 64x64 tile = 2x2 blocks of 16x16.  For v_mfma_*_16x16x4 lane l holds
-// A[l%16][l/16], B[l/16][l%16] and D[4*(l/16)+r][l%16], r = 0..3.
+// A[l%16][l/16] and B[l/16][l%16]; accumulator r of lane l is
+// D[l/16 + 4r][l%16] for the f64 instruction and D[4(l/16) + r][l%16] for
+// the f32 one (measured by rf_selftest_mfma_layout).
 __global__ void __launch_bounds__(256) lu_gemm_mfma_f64_kernel(double* A, int64_t lda, int n, int k0, int k1)
 {
     __shared__ double As[kTile][kKc + 1];
@@ -257,7 +259,7 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_f64_kernel(double* A, int64_
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = rbase + wr + i * 16 + 4 * (lane >> 4) + r;
+                const int row = rbase + wr + i * 16 + (lane >> 4) + 4 * r;
                 const int col = cbase + wc + j * 16 + (lane & 15);
                 if (row < n && col < n) {
                     double* p = A + (int64_t)col * lda + row;
@@ -428,7 +430,42 @@ hipError_t solve_impl(const T* LU, int64_t lda, int n, const int* ipiv, double* 
     return hipGetLastError();
 }
 
+// One wavefront: operands encode (mode 0) A[i][0] = i+1, B[0][j] = 1 -> D = i+1;
+// (mode 1) A[i][0] = 1, B[0][j] = j+1 -> D = j+1; (mode 2) only the lanes
+// 32..47 of A and B are 1 -> D = 1 where both use the same K slot.
+__global__ void mfma_probe_kernel(int mode, double* out64, float* out32)
+{
+    const int lane = threadIdx.x;
+    const int grp = lane >> 4, idx = lane & 15;
+    double a = 0.0, b = 0.0;
+    if (mode == 0) {
+        a = (grp == 0) ? idx + 1.0 : 0.0;
+        b = (grp == 0) ? 1.0 : 0.0;
+    } else if (mode == 1) {
+        a = (grp == 0) ? 1.0 : 0.0;
+        b = (grp == 0) ? idx + 1.0 : 0.0;
+    } else {
+        a = (grp == 2) ? 1.0 : 0.0;
+        b = (grp == 2) ? 1.0 : 0.0;
+    }
+    d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    f4 accf = f4{0.f, 0.f, 0.f, 0.f};
+    accf = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, accf, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        out64[lane * 4 + r] = acc[r];
+        out32[lane * 4 + r] = accf[r];
+    }
+}
+
 }  // namespace
+
+hipError_t mfma_probe(int mode, double* out64, float* out32, hipStream_t s)
+{
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, mode, out64, out32);
+    return hipGetLastError();
+}
 
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
                      hipStream_t s)

@@ -22,7 +22,7 @@ namespace rf {
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 
-int assemble_targets_per_block(int LQ);
+int assemble_targets_per_block(int LQ, int M);
 
 // fortran/quadrature.f90:6-96, fortran/matrix_assembly.f90:8-57,294-305,370
 void build_tables(int M, int noqi, double eps, Tables* t)
@@ -109,6 +109,14 @@ rf_sim::~rf_sim()
     if (h_dots) (void)hipHostFree(h_dots);
     if (h_y) (void)hipHostFree(h_y);
     for (auto& e : events) (void)hipEventDestroy(e);
+    for (auto& pr : ev_free) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    for (auto& t : ev_pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -189,15 +197,62 @@ double rf_sim::event_ms(hipEvent_t a, hipEvent_t b)
     return (double)ms;
 }
 
+int rf_sim::timed_pair(hipEvent_t* a, hipEvent_t* b)
+{
+    if (ev_pending.size() >= 256) {  // bound the pool: drain
+        HIP_TRY(hipStreamSynchronize(stream));
+        resolve_timed();
+    }
+    if (!ev_free.empty()) {
+        *a = ev_free.back().first;
+        *b = ev_free.back().second;
+        ev_free.pop_back();
+        return RF_OK;
+    }
+    HIP_TRY(hipEventCreate(a));
+    HIP_TRY(hipEventCreate(b));
+    return RF_OK;
+}
+
+void rf_sim::timed_push(hipEvent_t a, hipEvent_t b, int kind) { ev_pending.push_back({a, b, kind}); }
+
+// Call only when the stream is idle (after a synchronisation).
+void rf_sim::resolve_timed()
+{
+    for (const Timed& t : ev_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) ms = 0.f;
+        switch (t.kind) {
+        case kGemvA:
+            stats.gemv_ms_total += ms;
+            stats.gemv_launches += 1;
+            break;
+        case kAssemble:
+            stats.assemble_ms_total += ms;
+            stats.assemble_launches += 1;
+            break;
+        case kGemvW:
+            stats.velocity_gemv_ms_total += ms;
+            stats.velocity_gemv_launches += 1;
+            break;
+        case kLU:
+            stats.lu_ms_total += ms;
+            break;
+        }
+        ev_free.push_back({t.a, t.b});
+    }
+    ev_pending.clear();
+}
+
 int rf_sim::gemv_A(const double* x, double* y, const double* b, int mode)
 {
-    if (p.profile) HIP_TRY(hipEventRecord(ev_k0, stream));
-    HIP_TRY(launch_gemv(planA, d_A, x, d_gparts, y, b, mode, stream));
+    hipEvent_t a = nullptr, e = nullptr;
     if (p.profile) {
-        HIP_TRY(hipEventRecord(ev_k1, stream));
-        stats.gemv_ms_total += event_ms(ev_k0, ev_k1);
-        stats.gemv_launches += 1;
+        int rc = timed_pair(&a, &e);
+        if (rc) return rc;
     }
+    HIP_TRY(launch_gemv(planA, d_A, x, d_gparts, y, b, mode, stream, a, e));
+    if (p.profile) timed_push(a, e, kGemvA);
     return RF_OK;
 }
 
@@ -205,6 +260,7 @@ int rf_sim::fetch_dots(int count)
 {
     HIP_TRY(hipMemcpyAsync(h_dots, d_dots, sizeof(double) * count, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    resolve_timed();
     return RF_OK;
 }
 
@@ -382,19 +438,25 @@ int rf_sim::solve_lu(double* be_out)
         d_ipiv = static_cast<int*>(q);
         d_info = d_ipiv + n;
     }
-    if (p.profile) HIP_TRY(hipEventRecord(ev_k0, stream));
     HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
     HIP_TRY(hipMemsetAsync(d_info, 0, sizeof(int), stream));
     const int nb = (p.lu_block > 0) ? std::min(64, p.lu_block) : 64;
-    HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, 1, stream));
+    hipEvent_t ea = nullptr, eb = nullptr;
+    int rc0;
     if (p.profile) {
-        HIP_TRY(hipEventRecord(ev_k1, stream));
-        stats.lu_ms_total += event_ms(ev_k0, ev_k1);
+        if ((rc0 = timed_pair(&ea, &eb))) return rc0;
+        HIP_TRY(hipEventRecord(ea, stream));
+    }
+    HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, stream));
+    if (p.profile) {
+        HIP_TRY(hipEventRecord(eb, stream));
+        timed_push(ea, eb, kLU);
     }
     stats.lu_factorizations += 1;
     int info = 0;
     HIP_TRY(hipMemcpyAsync(h_dots, d_info, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    resolve_timed();
     std::memcpy(&info, h_dots, sizeof(int));
     if (info != 0) {
         rf::set_error("solve_lu: matrix is singular (U(" + std::to_string(info) + "," + std::to_string(info) +
@@ -441,7 +503,9 @@ int rf_sim::phase_begin()
 double rf_sim::phase_end()
 {
     if (hipEventRecord(ev_p1, stream) != hipSuccess) return 0.0;
-    return event_ms(ev_p0, ev_p1);
+    const double ms = event_ms(ev_p0, ev_p1);  // waits for the phase
+    resolve_timed();
+    return ms;
 }
 
 int rf_sim::assemble()
@@ -465,12 +529,15 @@ int rf_sim::assemble()
     a.W = d_W;
     a.bpart = d_bpart;
     a.mixed = mixed;
-    if (p.profile) HIP_TRY(hipEventRecord(ev_k0, stream));
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (p.profile) {
+        if ((rc = timed_pair(&ea, &eb))) return rc;
+        HIP_TRY(hipEventRecord(ea, stream));
+    }
     HIP_TRY(launch_assemble(tab, a, stream));
     if (p.profile) {
-        HIP_TRY(hipEventRecord(ev_k1, stream));
-        stats.assemble_ms_total += event_ms(ev_k0, ev_k1);
-        stats.assemble_launches += 1;
+        HIP_TRY(hipEventRecord(eb, stream));
+        timed_push(ea, eb, kAssemble);
     }
     HIP_TRY(launch_reduce_splits(d_bpart, asm_splits, n, n, d_b, stream));
     stats.ms_assemble = phase_end();
@@ -525,23 +592,21 @@ int rf_sim::solve()
 
 int rf_sim::update_velocities()
 {
+    int rc = RF_OK;
     if (!solved) {
         rf::set_error("rf_update_velocities: no solution for this configuration");
         return RF_EINVAL;
     }
-    int rc = phase_begin();
+    rc = phase_begin();
     if (rc) return rc;
     // old=new; new=3-old (fortran/advect_fibers.f90:465-466)
     vel_old = vel_new;
     vel_new = 1 - vel_old;
     HIP_TRY(launch_build_fext(N, M, d_force, d_x, d_fext, stream));
-    if (p.profile) HIP_TRY(hipEventRecord(ev_k0, stream));
-    HIP_TRY(launch_gemv(planW, d_W, d_fext, d_gparts, d_y6, nullptr, 0, stream));
-    if (p.profile) {
-        HIP_TRY(hipEventRecord(ev_k1, stream));
-        stats.velocity_gemv_ms_total += event_ms(ev_k0, ev_k1);
-        stats.velocity_gemv_launches += 1;
-    }
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (p.profile && (rc = timed_pair(&ea, &eb))) return rc;
+    HIP_TRY(launch_gemv(planW, d_W, d_fext, d_gparts, d_y6, nullptr, 0, stream, ea, eb));
+    if (p.profile) timed_push(ea, eb, kGemvW);
     HIP_TRY(launch_velocity_epilogue(N, tab.d, tab.e, d_pos[lvl_three], d_orient[lvl_three], d_force, d_y6,
                                      d_vel[vel_new], d_rot[vel_new], stream));
     stats.ms_velocities = phase_end();
@@ -629,7 +694,7 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
         set_error("rf_create: cannot query the device");
         return RF_EHIP;
     }
-    const int T = assemble_targets_per_block(s->LQ);
+    const int T = assemble_targets_per_block(s->LQ, s->M);
     const int tiles = (s->N + T - 1) / T;
     s->asm_splits = std::max(1, std::min(std::min(64, s->N), (4096 + tiles - 1) / tiles));
     const int es = s->mixed ? 4 : 8;
@@ -828,6 +893,72 @@ extern "C" int rf_matvec(rf_sim* s, const double* x, double* y)
     int rc = s->gemv_A(s->d_xc, s->d_r, nullptr, 0);
     if (rc) return rc;
     return copy_out(s, y, s->d_r, (size_t)s->n);
+}
+
+extern "C" int rf_synchronize(rf_sim* s)
+{
+    CHECK_SIM(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->resolve_timed();
+    return RF_OK;
+}
+
+extern "C" int rf_selftest_copy(int64_t bytes, int32_t reps, double* ms)
+{
+    if (bytes < 1024 || reps < 1 || !ms) {
+        set_error("rf_selftest_copy: bad arguments");
+        return RF_EINVAL;
+    }
+    bytes = bytes / 32 * 32;
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipMalloc(&src, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dst, bytes);
+    if (e == hipSuccess) e = hipMemset(src, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    double total = 0.0;
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        e = hipEventRecord(a, nullptr);
+        if (e == hipSuccess) e = launch_copy(src, dst, (size_t)bytes, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(b, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+        total += t;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e != hipSuccess) {
+        set_error(std::string("rf_selftest_copy: ") + hipGetErrorString(e));
+        return RF_EHIP;
+    }
+    *ms = total / reps;
+    return RF_OK;
+}
+
+extern "C" int rf_selftest_mfma_layout(int32_t mode, double* out64, float* out32)
+{
+    if (!out64 || !out32 || mode < 0 || mode > 2) {
+        set_error("rf_selftest_mfma_layout: bad arguments");
+        return RF_EINVAL;
+    }
+    double* d64 = nullptr;
+    float* d32 = nullptr;
+    HIP_TRY(hipMalloc(&d64, 256 * sizeof(double)));
+    hipError_t e = hipMalloc(&d32, 256 * sizeof(float));
+    if (e == hipSuccess) e = mfma_probe(mode, d64, d32, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out64, d64, 256 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out32, d32, 256 * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d64);
+    if (d32) (void)hipFree(d32);
+    if (e != hipSuccess) {
+        set_error(std::string("rf_selftest_mfma_layout: ") + hipGetErrorString(e));
+        return RF_EHIP;
+    }
+    return RF_OK;
 }
 
 extern "C" int32_t rf_num_fibers(const rf_sim* s) { return s ? s->N : 0; }

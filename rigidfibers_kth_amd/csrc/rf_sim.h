@@ -66,6 +66,19 @@ struct rf_sim {
 
     rf_stats stats{};
 
+    // per-kernel timing without synchronisation: events are recorded around
+    // launches and resolved at the next point where the stream is idle.
+    enum TimedKind { kGemvA = 0, kAssemble = 1, kGemvW = 2, kLU = 3 };
+    struct Timed {
+        hipEvent_t a, b;
+        int kind;
+    };
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_free;
+    std::vector<Timed> ev_pending;
+    int timed_pair(hipEvent_t* a, hipEvent_t* b);
+    void timed_push(hipEvent_t a, hipEvent_t b, int kind);
+    void resolve_timed();
+
     ~rf_sim();
     int alloc_all();
     int assemble();

@@ -53,12 +53,20 @@ __device__ __forceinline__ int sym6(int a, int b)
     return (a == b) ? a : (a + b + 2);  // (0,1)->3 (0,2)->4 (1,2)->5
 }
 
-// Analytic inner integrals G_0..G_M at point X against fiber (xj,tj).
-// R is the arithmetic type of the moment recurrences (double, or float for
-// the far field in mixed precision).
+// Moments of the analytic inner integral at point X against fiber (xj,tj):
+// I_n, J_n, S_n = integral over s in [-1,1] of s^n / |R|^{1,3,5}, n = 0..M+2,
+// R = X - xj - s tj (fortran/Int_analytical.f90:5-207).  R is the arithmetic
+// type of the recurrences (double, or float for the far field in mixed
+// precision); the moments are kept as double.
+template <int M>
+struct Moments {
+    double r0x, r0y, r0z, px, py, pz;
+    double I[M + 3], J[M + 3], S[M + 3];
+};
+
 template <int M, typename R>
-__device__ __forceinline__ void analytic_G(double X0, double X1, double X2, const double* xj,
-                                           const double* tj, double eps, double (&G)[(M + 1) * 6])
+__device__ __forceinline__ void analytic_moments(double X0, double X1, double X2, const double* xj, const double* tj,
+                                                 Moments<M>& mo)
 {
     constexpr int NM = M + 3;  // moments 0..M+2
     const R r0x = (R)(X0 - xj[0]), r0y = (R)(X1 - xj[1]), r0z = (R)(X2 - xj[2]);
@@ -121,36 +129,51 @@ __device__ __forceinline__ void analytic_G(double X0, double X1, double X2, cons
             s1 = sm;
         }
     }
-    const R e2 = R(2.0 * eps * eps), e6 = R(6.0 * eps * eps);
+    mo.r0x = (double)r0x;
+    mo.r0y = (double)r0y;
+    mo.r0z = (double)r0z;
+    mo.px = (double)px;
+    mo.py = (double)py;
+    mo.pz = (double)pz;
 #pragma unroll
-    for (int l = 0; l <= M; ++l) {
-        R L11 = 0, L12 = 0, L13 = 0, L14 = 0, L22 = 0, L23 = 0, L24 = 0;
-#pragma unroll
-        for (int m = 0; m <= l; ++m) {
-            if (kLeg[l][m] != 0.0) {
-                const R cf = R(kLeg[l][m]);
-                L11 += cf * I[m];
-                L12 += cf * J[m];
-                L13 += cf * J[m + 1];
-                L14 += cf * J[m + 2];
-                L22 += cf * S[m];
-                L23 += cf * S[m + 1];
-                L24 += cf * S[m + 2];
-            }
-        }
-        // G = al I + be R0 R0^T + ga (t R0^T + R0 t^T) + de t t^T
-        // (Int_analytical.f90:735-736 regrouped)
-        const R al = L11 + e2 * L12;
-        const R be = L12 - e6 * L22;
-        const R ga = e6 * L23 - L13;
-        const R de = L14 - e6 * L24;
-        G[l * 6 + 0] = (double)(al + be * r0x * r0x + R(2) * ga * px * r0x + de * px * px);
-        G[l * 6 + 1] = (double)(al + be * r0y * r0y + R(2) * ga * py * r0y + de * py * py);
-        G[l * 6 + 2] = (double)(al + be * r0z * r0z + R(2) * ga * pz * r0z + de * pz * pz);
-        G[l * 6 + 3] = (double)(be * r0x * r0y + ga * (px * r0y + r0x * py) + de * px * py);
-        G[l * 6 + 4] = (double)(be * r0x * r0z + ga * (px * r0z + r0x * pz) + de * px * pz);
-        G[l * 6 + 5] = (double)(be * r0y * r0z + ga * (py * r0z + r0y * pz) + de * py * pz);
+    for (int q = 0; q < NM; ++q) {
+        mo.I[q] = (double)I[q];
+        mo.J[q] = (double)J[q];
+        mo.S[q] = (double)S[q];
     }
+}
+
+// G_l from the moments: the Legendre combinations L11..L24 of
+// Int_analytical.f90:213-288 and the tensor of :735-736, regrouped as
+// G = al I + be R0 R0^T + ga (t R0^T + R0 t^T) + de t t^T.
+template <int M>
+__device__ __forceinline__ void analytic_Gl(const Moments<M>& mo, int l, double e2, double e6, double (&g)[6])
+{
+    double L11 = 0, L12 = 0, L13 = 0, L14 = 0, L22 = 0, L23 = 0, L24 = 0;
+#pragma unroll
+    for (int m = 0; m <= kMaxTerms; ++m) {
+        if (m <= l && kLeg[l][m] != 0.0) {
+            const double cf = kLeg[l][m];
+            L11 += cf * mo.I[m];
+            L12 += cf * mo.J[m];
+            L13 += cf * mo.J[m + 1];
+            L14 += cf * mo.J[m + 2];
+            L22 += cf * mo.S[m];
+            L23 += cf * mo.S[m + 1];
+            L24 += cf * mo.S[m + 2];
+        }
+    }
+    const double al = L11 + e2 * L12;
+    const double be = L12 - e6 * L22;
+    const double ga = e6 * L23 - L13;
+    const double de = L14 - e6 * L24;
+    const double rx = mo.r0x, ry = mo.r0y, rz = mo.r0z, px = mo.px, py = mo.py, pz = mo.pz;
+    g[0] = al + be * rx * rx + 2.0 * ga * px * rx + de * px * px;
+    g[1] = al + be * ry * ry + 2.0 * ga * py * ry + de * py * py;
+    g[2] = al + be * rz * rz + 2.0 * ga * pz * rz + de * pz * pz;
+    g[3] = be * rx * ry + ga * (px * ry + rx * py) + de * px * py;
+    g[4] = be * rx * rz + ga * (px * rz + rx * pz) + de * px * pz;
+    g[5] = be * ry * rz + ga * (py * rz + ry * pz) + de * py * pz;
 }
 
 // Numeric inner integrals (fortran/quadrature.f90:121-155): LQ-point sum of
@@ -207,21 +230,32 @@ struct KArgs {
 
 constexpr int kMaxItems = 8;  // A-row items per thread (3M/LQ <= 8)
 
+template <int M, bool ANALYTIC>
+__host__ __device__ constexpr int chunk_terms()
+{
+    // Legendre indices l per LDS stage: the analytic path keeps the moments
+    // in registers and stages G_l in two halves; the numeric path stages all.
+    return ANALYTIC ? (M + 2) / 2 : M + 1;
+}
+
 template <int M, bool ANALYTIC, bool MIXED>
-__global__ void __launch_bounds__(256) assemble_kernel(const KArgs a)
+__global__ void __launch_bounds__(256, 2) assemble_kernel(const KArgs a)
 {
     using AT = typename std::conditional<MIXED, float, double>::type;
     constexpr int NL = M + 1;
+    constexpr int CH = chunk_terms<M, ANALYTIC>();
+    constexpr int NCH = (NL + CH - 1) / CH;
     extern __shared__ double smem[];
     const int T = a.T, LQ = a.LQ, N = a.N;
     const int gstride = LQ + 1;
-    double* sG = smem;                              // [T*NL*6][LQ+1]
-    double* sTh = smem + (size_t)T * NL * 6 * gstride;  // [T][NL k][NL l][6]
+    double* sG = smem;                                    // [T*CH*6][LQ+1]
+    double* sTh = smem + (size_t)T * CH * 6 * gstride;    // [T][NL k][NL l][6]
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int i0 = blockIdx.x * T;
     const int jb = blockIdx.y * a.src_per_split;
     const int je = min(N, jb + a.src_per_split);
+    const double e2 = 2.0 * a.eps * a.eps, e6 = 6.0 * a.eps * a.eps;
 
     // phase-1 identity: target t1, point p1
     const int t1 = tid / LQ, p1 = tid - (tid / LQ) * LQ;
@@ -243,51 +277,71 @@ __global__ void __launch_bounds__(256) assemble_kernel(const KArgs a)
     for (int j = jb; j < je; ++j) {
         const double xj[3] = {a.pos[3 * j], a.pos[3 * j + 1], a.pos[3 * j + 2]};
         const double tj[3] = {a.orient[3 * j], a.orient[3 * j + 1], a.orient[3 * j + 2]};
-        // ---- phase 1: G_0..G_M at (t1, p1) ----
-        if (t1 < T) {
-            double G[NL * 6];
-            if (on1 && i1 != j) {
-                bool fp64 = true;
-                if (MIXED) {
-                    const double dx = X0 - xj[0], dy = X1 - xj[1], dz = X2 - xj[2];
-                    fp64 = (dx * dx + dy * dy + dz * dz) < a.near_c;
-                }
-                if (ANALYTIC) {
-                    if (fp64)
-                        analytic_G<M, double>(X0, X1, X2, xj, tj, a.eps, G);
-                    else
-                        analytic_G<M, float>(X0, X1, X2, xj, tj, a.eps, G);
-                } else {
-                    if (fp64)
-                        numeric_G<M, double>(X0, X1, X2, xj, tj, a.eps, LQ, a.pv, a.wp, G);
-                    else
-                        numeric_G<M, float>(X0, X1, X2, xj, tj, a.eps, LQ, a.pv, a.wp, G);
-                }
+        const bool act = on1 && (i1 != j);
+        // ---- phase 1a: per-point work shared by all l ----
+        Moments<M> mo;
+        double Gn[ANALYTIC ? 1 : NL * 6];
+        if (act) {
+            bool fp64 = true;
+            if (MIXED) {
+                const double dx = X0 - xj[0], dy = X1 - xj[1], dz = X2 - xj[2];
+                fp64 = (dx * dx + dy * dy + dz * dz) < a.near_c;
+            }
+            if constexpr (ANALYTIC) {
+                if (fp64)
+                    analytic_moments<M, double>(X0, X1, X2, xj, tj, mo);
+                else
+                    analytic_moments<M, float>(X0, X1, X2, xj, tj, mo);
             } else {
-#pragma unroll
-                for (int q = 0; q < NL * 6; ++q) G[q] = 0.0;
+                if (fp64)
+                    numeric_G<M, double>(X0, X1, X2, xj, tj, a.eps, LQ, a.pv, a.wp, Gn);
+                else
+                    numeric_G<M, float>(X0, X1, X2, xj, tj, a.eps, LQ, a.pv, a.wp, Gn);
             }
-#pragma unroll
-            for (int q = 0; q < NL * 6; ++q) sG[(size_t)(t1 * NL * 6 + q) * gstride + p1] = G[q];
         }
-        __syncthreads();
-        // ---- phase 2: Theta_{k l c} = sum_p w_p P_k(s_p) G_{l c}(p) ----
-        for (int it = tid; it < T * NL * 6; it += nthr) {
-            const double* g = sG + (size_t)it * gstride;
-            double acc[NL];
 #pragma unroll
-            for (int k = 0; k < NL; ++k) acc[k] = 0.0;
-            for (int p = 0; p < LQ; ++p) {
-                const double gp = g[p];
+        for (int ch = 0; ch < NCH; ++ch) {
+            // ---- phase 1b: G_l for the l of this stage into LDS ----
+            if (t1 < T) {
 #pragma unroll
-                for (int k = 0; k < NL; ++k) acc[k] += a.wp[k * LQ + p] * gp;
+                for (int li = 0; li < CH; ++li) {
+                    const int l = ch * CH + li;
+                    if (l <= M) {
+                        double g[6] = {0, 0, 0, 0, 0, 0};
+                        if (act) {
+                            if constexpr (ANALYTIC) {
+                                analytic_Gl<M>(mo, l, e2, e6, g);
+                            } else {
+#pragma unroll
+                                for (int q = 0; q < 6; ++q) g[q] = Gn[l * 6 + q];
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < 6; ++q) sG[(size_t)((t1 * CH + li) * 6 + q) * gstride + p1] = g[q];
+                    }
+                }
             }
-            const int t = it / (NL * 6), lc = it - t * (NL * 6);
-            const int l = lc / 6, c = lc - l * 6;
+            __syncthreads();
+            // ---- phase 2: Theta_{k l c} = sum_p w_p P_k(s_p) G_{l c}(p) ----
+            for (int it = tid; it < T * CH * 6; it += nthr) {
+                const int t = it / (CH * 6), rem = it - t * (CH * 6);
+                const int li = rem / 6, c = rem - li * 6;
+                const int l = ch * CH + li;
+                if (l > M) continue;
+                const double* gr = sG + (size_t)it * gstride;
+                double acc[NL];
 #pragma unroll
-            for (int k = 0; k < NL; ++k) sTh[((t * NL + k) * NL + l) * 6 + c] = acc[k];
+                for (int k = 0; k < NL; ++k) acc[k] = 0.0;
+                for (int p = 0; p < LQ; ++p) {
+                    const double gp = gr[p];
+#pragma unroll
+                    for (int k = 0; k < NL; ++k) acc[k] += a.wp[k * LQ + p] * gp;
+                }
+#pragma unroll
+                for (int k = 0; k < NL; ++k) sTh[((t * NL + k) * NL + l) * 6 + c] = acc[k];
+            }
+            __syncthreads();
         }
-        __syncthreads();
         // ---- phase 3: A rows, W rows, rhs partials ----
         int slot = 0;
         for (int it = tid; it < nItems; it += nthr, ++slot) {
@@ -300,7 +354,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const KArgs a)
                 const int64_t row = (int64_t)i * 3 * M + rem;
                 AT* A = reinterpret_cast<AT*>(a.A);
                 if (i == j) {
-#pragma unroll
+#pragma unroll 1
                     for (int l = 1; l <= M; ++l)
 #pragma unroll
                         for (int bc = 0; bc < 3; ++bc) {
@@ -311,7 +365,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const KArgs a)
                 }
                 const double* th = sTh + (size_t)(t * NL + k) * NL * 6;
                 const double Ek = a.Ek[k], gk = a.gam[k];
-#pragma unroll
+#pragma unroll 1
                 for (int l = 1; l <= M; ++l) {
                     const double* tl = th + l * 6;
 #pragma unroll
@@ -345,7 +399,7 @@ __global__ void __launch_bounds__(256) assemble_kernel(const KArgs a)
                 const int64_t row = (int64_t)i * 6 + r6;
                 AT* W = reinterpret_cast<AT*>(a.W);
                 const double* th = sTh + (size_t)(t * NL + r) * NL * 6;
-#pragma unroll
+#pragma unroll 1
                 for (int l = 0; l <= M; ++l)
 #pragma unroll
                     for (int c2 = 0; c2 < 3; ++c2) {
@@ -386,28 +440,29 @@ hipError_t launch_m(const KArgs& k, dim3 grid, int threads, size_t lds, bool ana
 
 }  // namespace
 
-size_t assemble_lds_bytes(int T, int LQ, int M)
+size_t assemble_lds_bytes(int T, int LQ, int M, bool analytic)
 {
     const size_t NL = M + 1;
-    return sizeof(double) * ((size_t)T * NL * 6 * (LQ + 1) + (size_t)T * NL * NL * 6);
+    const size_t CH = analytic ? (M + 2) / 2 : M + 1;
+    return sizeof(double) * ((size_t)T * CH * 6 * (LQ + 1) + (size_t)T * NL * NL * 6);
 }
 
-int assemble_targets_per_block(int LQ, int M)
+int assemble_targets_per_block(int LQ, int M, bool analytic)
 {
     // T*LQ threads; prefer a whole number of 64-wide wavefronts <= 256 whose
     // LDS fits in the 160 KiB of a CU.
     const size_t cap = 160 * 1024;
     for (int thr = 256; thr >= 64; thr -= 64)
-        if (thr % LQ == 0 && assemble_lds_bytes(thr / LQ, LQ, M) <= cap) return thr / LQ;
+        if (thr % LQ == 0 && assemble_lds_bytes(thr / LQ, LQ, M, analytic) <= cap) return thr / LQ;
     int T = std::max(1, 256 / LQ);
-    while (T > 1 && assemble_lds_bytes(T, LQ, M) > cap) --T;
+    while (T > 1 && assemble_lds_bytes(T, LQ, M, analytic) > cap) --T;
     return T;
 }
 
 hipError_t launch_assemble(const Tables& t, const AssembleArgs& a, hipStream_t s)
 {
     const int M = t.M, LQ = t.LQ;
-    const int T = a.targets_per_block > 0 ? a.targets_per_block : assemble_targets_per_block(LQ, M);
+    const int T = a.targets_per_block > 0 ? a.targets_per_block : assemble_targets_per_block(LQ, M, a.analytic != 0);
     const int threads = T * LQ;
     if (threads > 1024 || T * 3 * M > kMaxItems * threads) return hipErrorInvalidValue;
     KArgs k;
@@ -435,7 +490,7 @@ hipError_t launch_assemble(const Tables& t, const AssembleArgs& a, hipStream_t s
     k.A = a.A;
     k.W = a.W;
     k.bpart = a.bpart;
-    const size_t lds = assemble_lds_bytes(T, LQ, M);
+    const size_t lds = assemble_lds_bytes(T, LQ, M, a.analytic != 0);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     dim3 grid((a.N + T - 1) / T, a.splits);
     const bool an = a.analytic != 0, mx = a.mixed != 0;

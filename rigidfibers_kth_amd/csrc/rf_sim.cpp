@@ -22,7 +22,7 @@ namespace rf {
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 
-int assemble_targets_per_block(int LQ, int M);
+int assemble_targets_per_block(int LQ, int M, bool analytic);
 
 // fortran/quadrature.f90:6-96, fortran/matrix_assembly.f90:8-57,294-305,370
 void build_tables(int M, int noqi, double eps, Tables* t)
@@ -694,7 +694,7 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
         set_error("rf_create: cannot query the device");
         return RF_EHIP;
     }
-    const int T = assemble_targets_per_block(s->LQ, s->M);
+    const int T = assemble_targets_per_block(s->LQ, s->M, P.use_analytical_integration != 0);
     const int tiles = (s->N + T - 1) / T;
     s->asm_splits = std::max(1, std::min(std::min(64, s->N), (4096 + tiles - 1) / tiles));
     const int es = s->mixed ? 4 : 8;

@@ -50,6 +50,10 @@ extern "C" {
 #define RF_WARM_EXTRAP 2  /* 2 x_n - x_{n-1} (linear extrapolation in time) */
 #define RF_WARM_LEARNED 3 /* x_{n} + MLP(now) - MLP(prev): learned increment */
 
+/* GMRES preconditioners (the reference runs GMRES without one) */
+#define RF_PRECOND_NONE 0
+#define RF_PRECOND_NEIGHBOUR 1 /* block-Jacobi on each fiber + its K nearest fibers (restricted additive Schwarz) */
+
 /* output number formats */
 #define RF_FMT_F16_8 0  /* '(*(F16.8))' as the Fortran's *.out / tests/reference *.ref */
 #define RF_FMT_F24_16 1 /* '(2F24.16)' as XcTData / VelRotData / Coeffvec */
@@ -85,7 +89,9 @@ typedef struct rf_params {
     int32_t lu_block;   /* LU panel width (0 = default) */
     double mixed_inner_tol; /* mixed mode: tolerance of the FP32-operator solve (0 = default 1e-10) */
     int32_t lu_valu_update; /* 1: LU trailing update on the vector ALUs instead of the matrix cores */
-    int32_t reserved[7];
+    int32_t precond;        /* RF_PRECOND_*: right preconditioner of GMRES */
+    int32_t precond_neighbours; /* K nearest fibers per preconditioner cluster (1..6, 0 = 4) */
+    int32_t reserved[5];
 } rf_params;
 
 typedef struct rf_stats {
@@ -120,7 +126,8 @@ void rf_params_default(rf_params *p);
 
 /* v1 parameter file (src/parameters.cc:124-180; fortran/advect_fibers.f90:182-222).
  * Lines after the last mandatory one may hold "key value" extensions:
- * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1. */
+ * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1,
+ * precond none|neighbour, precond_neighbours K. */
 int rf_parse_parameters(const char *path, rf_params *out);
 
 /* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per
@@ -158,6 +165,15 @@ int rf_get_matrix(const rf_sim *sim, double *A, int64_t lda);
 int rf_get_stats(const rf_sim *sim, rf_stats *stats);
 /* Sets the coefficient vector (used as the next GMRES initial guess). */
 int rf_set_coefficients(rf_sim *sim, const double *x);
+
+/* The k nearest fibers of every fiber by segment-segment distance at the
+ * current configuration (ascending; nbr and dist hold N*k entries).  Needs
+ * k <= the neighbours the handle keeps (precond_neighbours, or 16 with the
+ * learned warm start). No counterpart in the reference. */
+int rf_get_neighbours(rf_sim *sim, int32_t k, int32_t *nbr, double *dist);
+/* z = M^-1 r with the neighbour preconditioner built by the last rf_solve
+ * (n doubles each).  No counterpart in the reference. */
+int rf_precond_apply(rf_sim *sim, const double *r, double *z);
 
 /* Waits for all work queued on the handle's stream. */
 int rf_synchronize(rf_sim *sim);

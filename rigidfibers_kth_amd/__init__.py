@@ -26,6 +26,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 WARM_PREV, WARM_ZERO, WARM_EXTRAP, WARM_LEARNED = 0, 1, 2, 3
 FMT_F16_8, FMT_F24_16 = 0, 1
 SOLVER_GMRES, SOLVER_LU, SOLVER_LU_FALLBACK = 0, 1, 2
+PRECOND_NONE, PRECOND_NEIGHBOUR = 0, 1
 
 
 class rf_params(ctypes.Structure):
@@ -52,7 +53,9 @@ class rf_params(ctypes.Structure):
         ("lu_block", ctypes.c_int32),
         ("mixed_inner_tol", ctypes.c_double),
         ("lu_valu_update", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("precond", ctypes.c_int32),
+        ("precond_neighbours", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 
@@ -124,6 +127,8 @@ _SIGNATURES = {
     "rf_set_coefficients": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "rf_matvec": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "rf_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "rf_get_neighbours": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]),
+    "rf_precond_apply": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "rf_selftest_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _dp]),
     "rf_write_step_outputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]),
     "rf_selftest_mfma_layout": (ctypes.c_int, [ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_float)]),
@@ -317,6 +322,19 @@ class Simulation:
         y = np.empty(self.n)
         _check(lib().rf_matvec(self._h, _p(x), _p(y)), "rf_matvec")
         return y
+
+    def neighbours(self, k):
+        nbr = np.empty(self.N * k, dtype=np.int32)
+        dist = np.empty(self.N * k)
+        _check(lib().rf_get_neighbours(self._h, k, nbr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _p(dist)),
+               "rf_get_neighbours")
+        return nbr.reshape(self.N, k), dist.reshape(self.N, k)
+
+    def precond_apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty(self.n)
+        _check(lib().rf_precond_apply(self._h, _p(r), _p(z)), "rf_precond_apply")
+        return z
 
     def synchronize(self):
         _check(lib().rf_synchronize(self._h), "rf_synchronize")

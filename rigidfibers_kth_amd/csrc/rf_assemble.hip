@@ -55,71 +55,76 @@ __device__ __forceinline__ int sym6(int a, int b)
 
 // Moments of the analytic inner integral at point X against fiber (xj,tj):
 // I_n, J_n, S_n = integral over s in [-1,1] of s^n / |R|^{1,3,5}, n = 0..M+2,
-// R = X - xj - s tj (fortran/Int_analytical.f90:5-207).  R is the arithmetic
-// type of the recurrences (double, or float for the far field in mixed
-// precision); the moments are kept as double.
+// R = X - xj - s tj (fortran/Int_analytical.f90:5-207).  The closed forms of
+// n = 0, 1 (logarithms, square roots, the 1/d terms) and the forward
+// recurrences of the near field (c < 10) are FP64; RB is the arithmetic type
+// of the 26-step backward recurrences of the far field (double, or float in
+// mixed precision, whose error is then ~FP32 rounding of the moments).
 template <int M>
 struct Moments {
     double r0x, r0y, r0z, px, py, pz;
     double I[M + 3], J[M + 3], S[M + 3];
 };
 
-template <int M, typename R>
+template <int M, typename RB>
 __device__ __forceinline__ void analytic_moments(double X0, double X1, double X2, const double* xj, const double* tj,
                                                  Moments<M>& mo)
 {
     constexpr int NM = M + 3;  // moments 0..M+2
-    const R r0x = (R)(X0 - xj[0]), r0y = (R)(X1 - xj[1]), r0z = (R)(X2 - xj[2]);
-    const R px = (R)tj[0], py = (R)tj[1], pz = (R)tj[2];
-    const R b = R(-2) * (px * r0x + py * r0y + pz * r0z);
-    const R c = r0x * r0x + r0y * r0y + r0z * r0z;
-    const R ue = sqrt(R(1) + b + c);
-    const R ub = sqrt(R(1) - b + c);
-    const R d = c - R(0.25) * b * b;
-    R I[NM], J[NM], S[NM];
-    I[0] = log(fabs(R(2) + b + R(2) * ue)) - log(fabs(R(-2) + b + R(2) * ub));
-    I[1] = ue - ub - R(0.5) * b * I[0];
-    const R iue = R(1) / ue, iub = R(1) / ub;
-    if (d < R(1e-14)) {  // collinear (Int_analytical.f90:162-169,189-196)
-        const R sp = R(1) + R(0.5) * b, sm = R(-1) + R(0.5) * b;
-        J[0] = R(-0.5) / (sp * sp) + R(0.5) / (sm * sm);
-        J[1] = -iue + iub - R(0.5) * b * J[0];
-        const R ap = R(2) + b, am = R(-2) + b;
-        S[0] = R(-4) / (ap * ap * ap * ap) + R(4) / (am * am * am * am);
-        S[1] = R(-8) / (R(3) * ap * ap * ap) + R(8) / (R(3) * am * am * am) - R(0.5) * b * S[0];
+    const double r0x = X0 - xj[0], r0y = X1 - xj[1], r0z = X2 - xj[2];
+    const double px = tj[0], py = tj[1], pz = tj[2];
+    const double b = -2.0 * (px * r0x + py * r0y + pz * r0z);
+    const double c = r0x * r0x + r0y * r0y + r0z * r0z;
+    const double ue = sqrt(1.0 + b + c);
+    const double ub = sqrt(1.0 - b + c);
+    const double d = c - 0.25 * b * b;
+    double* I = mo.I;
+    double* J = mo.J;
+    double* S = mo.S;
+    I[0] = log(fabs(2.0 + b + 2.0 * ue)) - log(fabs(-2.0 + b + 2.0 * ub));
+    I[1] = ue - ub - 0.5 * b * I[0];
+    const double iue = 1.0 / ue, iub = 1.0 / ub;
+    if (d < 1e-14) {  // collinear (Int_analytical.f90:162-169,189-196)
+        const double sp = 1.0 + 0.5 * b, sm = -1.0 + 0.5 * b;
+        J[0] = -0.5 / (sp * sp) + 0.5 / (sm * sm);
+        J[1] = -iue + iub - 0.5 * b * J[0];
+        const double ap = 2.0 + b, am = -2.0 + b;
+        S[0] = -4.0 / (ap * ap * ap * ap) + 4.0 / (am * am * am * am);
+        S[1] = -8.0 / (3.0 * ap * ap * ap) + 8.0 / (3.0 * am * am * am) - 0.5 * b * S[0];
     } else {
-        const R id = R(1) / d;
-        J[0] = ((R(1) + R(0.5) * b) * iue - (R(-1) + R(0.5) * b) * iub) * id;
-        J[1] = -iue + iub - R(0.5) * b * J[0];
-        const R iue3 = iue * iue * iue, iub3 = iub * iub * iub;
-        const R i6d = id * R(1.0 / 6.0);
-        S[0] = ((R(2) + b) * iue3 - (R(-2) + b) * iub3) * i6d + (R(2.0 / 3.0) * id) * J[0];
-        S[1] = (-(b + R(2) * c) * iue3 + (R(2) * c - b) * iub3) * i6d - (b * id * R(1.0 / 3.0)) * J[0];
+        const double id = 1.0 / d;
+        J[0] = ((1.0 + 0.5 * b) * iue - (-1.0 + 0.5 * b) * iub) * id;
+        J[1] = -iue + iub - 0.5 * b * J[0];
+        const double iue3 = iue * iue * iue, iub3 = iub * iub * iub;
+        const double i6d = id * (1.0 / 6.0);
+        S[0] = ((2.0 + b) * iue3 - (-2.0 + b) * iub3) * i6d + ((2.0 / 3.0) * id) * J[0];
+        S[1] = (-(b + 2.0 * c) * iue3 + (2.0 * c - b) * iub3) * i6d - (b * id * (1.0 / 3.0)) * J[0];
     }
-    if (c < R(10)) {
+    if (c < 10.0) {
         // forward recurrences (Int_analytical.f90:123-136,172-176,197-200)
 #pragma unroll
         for (int n = 2; n < NM; ++n) {
-            const R sgn = (n % 2 == 1) ? R(1) : R(-1);  // (-1)^(n-1)
-            I[n] = (ue - sgn * ub) * R(1.0 / n) - R((n - 0.5) / n) * b * I[n - 1] - R((n - 1.0) / n) * c * I[n - 2];
+            const double sgn = (n % 2 == 1) ? 1.0 : -1.0;  // (-1)^(n-1)
+            I[n] = (ue - sgn * ub) * (1.0 / n) - ((n - 0.5) / n) * b * I[n - 1] - ((n - 1.0) / n) * c * I[n - 2];
             J[n] = I[n - 2] - b * J[n - 1] - c * J[n - 2];
             S[n] = J[n - 2] - b * S[n - 1] - c * S[n - 2];
         }
     } else {
         // backward recurrences from n = 27 with zero start
         // (Int_analytical.f90:137-158,177-183,201-206)
-        const R ic = R(1) / c;
-        R i1 = 0, i2 = 0, j1 = 0, j2 = 0, s1 = 0, s2 = 0;
+        const RB ic = (RB)(1.0 / c), rb = (RB)b;
+        const RB uem = (RB)(ue - ub), uep = (RB)(ue + ub);
+        RB i1 = 0, i2 = 0, j1 = 0, j2 = 0, s1 = 0, s2 = 0;
 #pragma unroll
         for (int m = 27; m >= 2; --m) {
-            const R sgn = ((m + 1) % 2 == 0) ? R(1) : R(-1);  // (-1)^(m+1)
-            const R im = ic * R(1.0 / (m + 1.0)) * ((ue - sgn * ub) - R(m + 1.5) * b * i1 - R(m + 2.0) * i2);
-            const R jm = ic * (im - b * j1 - j2);
-            const R sm = ic * (jm - b * s1 - s2);
+            const RB base = ((m + 1) % 2 == 0) ? uem : uep;  // ue - (-1)^(m+1) ub
+            const RB im = ic * RB(1.0 / (m + 1.0)) * (base - RB(m + 1.5) * rb * i1 - RB(m + 2.0) * i2);
+            const RB jm = ic * (im - rb * j1 - j2);
+            const RB sm = ic * (jm - rb * s1 - s2);
             if (m < NM) {
-                I[m] = im;
-                J[m] = jm;
-                S[m] = sm;
+                I[m] = (double)im;
+                J[m] = (double)jm;
+                S[m] = (double)sm;
             }
             i2 = i1;
             i1 = im;
@@ -129,18 +134,12 @@ __device__ __forceinline__ void analytic_moments(double X0, double X1, double X2
             s1 = sm;
         }
     }
-    mo.r0x = (double)r0x;
-    mo.r0y = (double)r0y;
-    mo.r0z = (double)r0z;
-    mo.px = (double)px;
-    mo.py = (double)py;
-    mo.pz = (double)pz;
-#pragma unroll
-    for (int q = 0; q < NM; ++q) {
-        mo.I[q] = (double)I[q];
-        mo.J[q] = (double)J[q];
-        mo.S[q] = (double)S[q];
-    }
+    mo.r0x = r0x;
+    mo.r0y = r0y;
+    mo.r0z = r0z;
+    mo.px = px;
+    mo.py = py;
+    mo.pz = pz;
 }
 
 // G_l from the moments: the Legendre combinations L11..L24 of

@@ -158,6 +158,12 @@ extern "C" int rf_parse_parameters(const char* path, rf_params* out)
             ok = parse_int(val, &p.profile);
         } else if (key == "lu_block") {
             ok = parse_int(val, &p.lu_block);
+        } else if (key == "precond") {
+            if (val == "none") p.precond = RF_PRECOND_NONE;
+            else if (val == "neighbour") p.precond = RF_PRECOND_NEIGHBOUR;
+            else ok = false;
+        } else if (key == "precond_neighbours") {
+            ok = parse_int(val, &p.precond_neighbours);
         } else if (key == "mixed_inner_tol") {
             ok = parse_double(val, &p.mixed_inner_tol);
         } else {

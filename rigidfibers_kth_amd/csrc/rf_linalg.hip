@@ -198,7 +198,7 @@ __global__ void vy_plus_x_kernel(const double* __restrict__ V, int64_t ldv, int 
     if (i >= n) return;
     double s = 0.0;
     for (int j = 0; j < k; ++j) s += V[(int64_t)j * ldv + i] * y[j];
-    xc[i] = x[i] + s;
+    xc[i] = x ? x[i] + s : s;
 }
 
 __global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, size_t n4)

@@ -15,6 +15,7 @@
 
 #include "rf_internal.h"
 #include "rf_lu.h"
+#include "rf_precond.h"
 #include "rf_sim.h"
 
 namespace rf {
@@ -307,6 +308,20 @@ static void rot1(double* x, double* y, double c, double s)
     *x = t;
 }
 
+// xc = x + V y (no preconditioner) or x + M^-1 (V y) (right preconditioner)
+int rf_sim::update_iterate(int k)
+{
+    if (!prec_ready) {
+        HIP_TRY(launch_vy_plus_x(d_V, lda, k, d_y, d_x, n, d_xc, stream));
+        return RF_OK;
+    }
+    HIP_TRY(launch_vy_plus_x(d_V, lda, k, d_y, nullptr, n, d_t, stream));
+    int rc = precond_apply(d_t, d_z);
+    if (rc) return rc;
+    HIP_TRY(launch_axpby(n, 1.0, d_x, 1.0, d_z, d_xc, stream));
+    return RF_OK;
+}
+
 // Restarted GMRES with the CERFACS dGMRES semantics the reference uses
 // (fortran/coeffs_and_vels.f90:253-339, fortran/dGMRES.f:736-1495):
 // no preconditioner, stop when the Arnoldi backward error |g_{j+1}|/||b||
@@ -314,7 +329,9 @@ static void rot1(double* x, double* y, double c, double s)
 // restart, d_x holds the initial guess.  Orthogonalisation is classical
 // Gram-Schmidt applied twice (one fused multi-dot per pass instead of j
 // sequential dots); in exact arithmetic it builds the same Krylov basis and
-// the same iterates as the reference's modified Gram-Schmidt.
+// the same iterates as the reference's modified Gram-Schmidt.  With
+// rf_params.precond the same iteration runs on A M^-1 (right preconditioning,
+// the stopping test still uses the true residual b - A x).
 int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
 {
     const int m = restart;
@@ -358,7 +375,12 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
         for (int jH = 0; jH < m; ++jH) {
             const int k = jH + 1;
             double* vj = d_V + (size_t)jH * ldv;
-            if ((rc = gemv_A(vj, d_w, nullptr, 0))) return rc;
+            if (prec_ready) {  // w = A M^-1 v_j
+                if ((rc = precond_apply(vj, d_z))) return rc;
+                if ((rc = gemv_A(d_z, d_w, nullptr, 0))) return rc;
+            } else {
+                if ((rc = gemv_A(vj, d_w, nullptr, 0))) return rc;
+            }
             HIP_TRY(launch_multidot(d_V, ldv, k, d_w, n, d_dparts, d_counter, D1, stream));
             HIP_TRY(launch_orth_update(d_V, ldv, k, D1, d_w, n, 1, d_dparts, d_counter, D2, stream));
             HIP_TRY(launch_orth_update(d_V, ldv, k, D2, d_w, n, 0, nullptr, nullptr, nullptr, stream));
@@ -380,7 +402,7 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
                 hess_solve(H, ldh, g, k, y);
                 std::copy(y.begin(), y.begin() + k, h_y);
                 HIP_TRY(hipMemcpyAsync(d_y, h_y, sizeof(double) * k, hipMemcpyHostToDevice, stream));
-                HIP_TRY(launch_vy_plus_x(d_V, ldv, k, d_y, d_x, n, d_xc, stream));
+                if ((rc = update_iterate(k))) return rc;
                 have_xc = true;
                 if ((rc = gemv_A(d_xc, d_r, d_b, 1))) return rc;
                 double rr;
@@ -400,7 +422,7 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
             hess_solve(H, ldh, g, m, y);
             std::copy(y.begin(), y.begin() + m, h_y);
             HIP_TRY(hipMemcpyAsync(d_y, h_y, sizeof(double) * m, hipMemcpyHostToDevice, stream));
-            HIP_TRY(launch_vy_plus_x(d_V, ldv, m, d_y, d_x, n, d_xc, stream));
+            if ((rc = update_iterate(m))) return rc;
         }
         HIP_TRY(hipMemcpyAsync(d_x, d_xc, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
         if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
@@ -495,6 +517,80 @@ int rf_sim::solve_lu(double* be_out)
     return RF_OK;
 }
 
+int rf_sim::setup_neighbours()
+{
+    if (knn < 1) return RF_OK;
+    if (!d_nbr) {
+        void* q = nullptr;
+        if (hipMalloc(&q, sizeof(int) * (size_t)N * knn) != hipSuccess) {
+            rf::set_error("cannot allocate the neighbour lists");
+            return RF_ENOMEM;
+        }
+        allocs.push_back(q);
+        d_nbr = static_cast<int*>(q);
+        if (hipMalloc(&q, sizeof(double) * (size_t)N * knn) != hipSuccess) {
+            rf::set_error("cannot allocate the neighbour distances");
+            return RF_ENOMEM;
+        }
+        allocs.push_back(q);
+        d_nbr_dist = static_cast<double*>(q);
+    }
+    HIP_TRY(launch_knn(N, knn, d_pos[lvl_three], d_orient[lvl_three], d_nbr, d_nbr_dist, stream));
+    return RF_OK;
+}
+
+// Neighbour-coupled block-Jacobi (restricted additive Schwarz) right
+// preconditioner; built from the assembled matrix of this step.
+int rf_sim::setup_precond()
+{
+    const int bs = 3 * M;
+    const int dim = bs * (ras_k + 1);
+    if (!d_B) {
+        void* q = nullptr;
+        if (hipMalloc(&q, sizeof(double) * (size_t)N * dim * bs) != hipSuccess) {
+            rf::set_error("cannot allocate the preconditioner");
+            return RF_ENOMEM;
+        }
+        allocs.push_back(q);
+        d_B = static_cast<double*>(q);
+        for (double** v : {&d_z, &d_t}) {
+            if (hipMalloc(&q, sizeof(double) * (size_t)lda) != hipSuccess) {
+                rf::set_error("cannot allocate preconditioner vectors");
+                return RF_ENOMEM;
+            }
+            allocs.push_back(q);
+            *v = static_cast<double*>(q);
+        }
+        if (hipMalloc(&q, sizeof(int)) != hipSuccess) {
+            rf::set_error("cannot allocate preconditioner flag");
+            return RF_ENOMEM;
+        }
+        allocs.push_back(q);
+        d_pinfo = static_cast<int*>(q);
+    }
+    int rc = setup_neighbours();
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(d_pinfo, 0, sizeof(int), stream));
+    HIP_TRY(launch_ras_setup(N, ras_k, knn, bs, d_nbr, d_A, mixed ? 4 : 8, lda, d_B, d_pinfo, stream));
+    int info = 0;
+    HIP_TRY(hipMemcpyAsync(h_dots, d_pinfo, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    resolve_timed();
+    std::memcpy(&info, h_dots, sizeof(int));
+    if (info != 0) {
+        rf::set_error("preconditioner: singular cluster matrix at fiber " + std::to_string(info - 1));
+        return RF_ENOCONV;
+    }
+    prec_ready = true;
+    return RF_OK;
+}
+
+int rf_sim::precond_apply(const double* r, double* z)
+{
+    HIP_TRY(launch_ras_apply(N, ras_k, knn, 3 * M, d_nbr, d_B, r, z, stream));
+    return RF_OK;
+}
+
 int rf_sim::phase_begin()
 {
     HIP_TRY(hipEventRecord(ev_p0, stream));
@@ -571,6 +667,8 @@ int rf_sim::solve()
         if (rc) return rc;
         stats.last_solver = RF_SOLVER_LU;
     } else {
+        prec_ready = false;
+        if (p.precond == RF_PRECOND_NEIGHBOUR && (rc = setup_precond())) return rc;
         rc = solve_gmres(&iters, &be, &status);
         if (rc) return rc;
         stats.last_solver = RF_SOLVER_GMRES;
@@ -636,6 +734,7 @@ int rf_sim::advance()
     ++steps_taken;
     stats.steps_taken = steps_taken;
     assembled = solved = have_velocities = false;
+    prec_ready = false;
     return RF_OK;
 }
 
@@ -666,8 +765,11 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     const rf_params& P = *params;
     if (num_fibers < 1 || P.num_terms < 1 || P.num_terms > kMaxTerms || P.num_quad_intervals < 1 ||
         3 * P.num_quad_intervals > kMaxQuad || !(P.slenderness > 0.0) || P.gmres_restart < 1 ||
-        P.gmres_restart > 60 || P.gmres_max_iters < 1 || (P.precision != RF_PREC_FP64 && P.precision != RF_PREC_MIXED)) {
-        set_error("rf_create: invalid parameters (need N>=1, 1<=M<=8, 1<=NoQI<=32, eps>0, 1<=restart<=60)");
+        P.gmres_restart > 60 || P.gmres_max_iters < 1 || (P.precision != RF_PREC_FP64 && P.precision != RF_PREC_MIXED) ||
+        P.precond < 0 || P.precond > 1 || P.precond_neighbours < 0 || P.precond_neighbours > 6 ||
+        (P.precond && num_fibers < 2)) {
+        set_error("rf_create: invalid parameters (need N>=1, 1<=M<=8, 1<=NoQI<=32, eps>0, 1<=restart<=60, "
+                  "precond 0|1 with 0<=precond_neighbours<=6 and N>=2)");
         return RF_EINVAL;
     }
     int dev = 0, ndev = 0;
@@ -683,6 +785,8 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     s->n = 3 * s->M * s->N;
     s->mixed = (P.precision == RF_PREC_MIXED) ? 1 : 0;
     s->restart = P.gmres_restart;
+    s->ras_k = P.precond ? std::min(num_fibers - 1, P.precond_neighbours > 0 ? P.precond_neighbours : 4) : 0;
+    s->knn = s->ras_k;
     build_tables(s->M, P.num_quad_intervals, P.slenderness, &s->tab);
     s->LQ = s->tab.LQ;
     s->lda = round_up(s->n, 256);
@@ -893,6 +997,41 @@ extern "C" int rf_matvec(rf_sim* s, const double* x, double* y)
     int rc = s->gemv_A(s->d_xc, s->d_r, nullptr, 0);
     if (rc) return rc;
     return copy_out(s, y, s->d_r, (size_t)s->n);
+}
+
+extern "C" int rf_get_neighbours(rf_sim* s, int32_t k, int32_t* nbr, double* dist)
+{
+    CHECK_SIM(s);
+    if (!nbr || !dist || k < 1 || k > s->knn) {
+        set_error("rf_get_neighbours: need 1 <= k <= " + std::to_string(s->knn));
+        return RF_EINVAL;
+    }
+    int rc = s->setup_neighbours();
+    if (rc) return rc;
+    std::vector<int> tn((size_t)s->N * s->knn);
+    std::vector<double> td((size_t)s->N * s->knn);
+    HIP_TRY(hipMemcpyAsync(tn.data(), s->d_nbr, sizeof(int) * tn.size(), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(td.data(), s->d_nbr_dist, sizeof(double) * td.size(), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < s->N; ++i)
+        for (int r = 0; r < k; ++r) {
+            nbr[(size_t)i * k + r] = tn[(size_t)i * s->knn + r];
+            dist[(size_t)i * k + r] = td[(size_t)i * s->knn + r];
+        }
+    return RF_OK;
+}
+
+extern "C" int rf_precond_apply(rf_sim* s, const double* r, double* z)
+{
+    CHECK_SIM(s);
+    if (!r || !z || !s->prec_ready) {
+        set_error("rf_precond_apply: no preconditioner (needs precond and a previous rf_solve)");
+        return RF_EINVAL;
+    }
+    HIP_TRY(hipMemcpyAsync(s->d_t, r, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    int rc = s->precond_apply(s->d_t, s->d_z);
+    if (rc) return rc;
+    return copy_out(s, z, s->d_z, (size_t)s->n);
 }
 
 extern "C" int rf_synchronize(rf_sim* s)

@@ -59,6 +59,16 @@ struct rf_sim {
     unsigned int* d_counter = nullptr;
     double* h_dots = nullptr;  // pinned
     double* h_y = nullptr;     // pinned
+    // neighbourhoods and preconditioner (allocated on first use)
+    int knn = 0;           // neighbours kept per fiber
+    int ras_k = 0;         // neighbours per preconditioner cluster
+    int* d_nbr = nullptr;  // [N][knn]
+    double* d_nbr_dist = nullptr;
+    double* d_B = nullptr;  // [N][3M(K+1)][3M]
+    double* d_z = nullptr;
+    double* d_t = nullptr;
+    int* d_pinfo = nullptr;
+    bool prec_ready = false;
     // LU workspace (allocated on first use)
     void* d_LU = nullptr;
     int* d_ipiv = nullptr;
@@ -91,6 +101,10 @@ struct rf_sim {
     int norm2(const double* v, double* out);
     int solve_gmres(int* iters, double* be, int* status);
     int solve_lu(double* be);
+    int setup_neighbours();
+    int setup_precond();
+    int precond_apply(const double* r, double* z);
+    int update_iterate(int k);
     int phase_begin();
     double phase_end();
     static double event_ms(hipEvent_t a, hipEvent_t b);

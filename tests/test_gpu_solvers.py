@@ -1,0 +1,124 @@
+"""Solver extensions beyond the reference, checked on the GPU through the C ABI:
+
+* fiber neighbourhoods (k nearest by segment-segment distance) vs numpy;
+* the neighbour-coupled block-Jacobi preconditioner vs a numpy restatement
+  (cluster = fiber + K nearest, rows of the cluster inverse), and GMRES with it
+  reaching the same solution as the reference's unpreconditioned GMRES;
+* mixed precision (FP32 matrix with FP64 near field, FP64 Arnoldi and
+  residuals): error against the FP64 path, FP32 LU with FP64 refinement.
+"""
+import numpy as np
+import pytest
+
+import rigidfibers_kth_amd as rf
+from bench import layout, segment_distances
+
+pytestmark = pytest.mark.gpu
+
+
+def _sim(pos, ori, **kw):
+    base = dict(gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-10)
+    base.update(kw)
+    return rf.Simulation(rf.Configuration(rf.default_params(**base), pos, ori))
+
+
+def _hard_state():
+    """The configuration after two steps of the volume-fraction 5e-2 dynamics
+    (N=300): sedimentation has brought fibers into near contact and the
+    reference's unpreconditioned GMRES(30) needs ~1000 iterations on it."""
+    pos, ori = layout(300, 5e-2, 0x5EED0003)
+    with _sim(pos, ori) as sim:
+        sim.step()
+        sim.step()
+        return sim.state()
+
+
+def test_neighbours_vs_numpy():
+    N, K = 180, 6
+    pos, ori = layout(N, 2e-2, 41)
+    with _sim(pos, ori, precond=1, precond_neighbours=K) as sim:
+        nbr, dist = sim.neighbours(K)
+    P, T = pos.reshape(-1, 3), ori.reshape(-1, 3)
+    for i in range(N):
+        d = segment_distances(P[i], T[i], P, T)
+        d[i] = np.inf
+        order = np.lexsort((np.arange(N), d))[:K]
+        np.testing.assert_array_equal(nbr[i], order)
+        np.testing.assert_allclose(dist[i], d[order], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("precision", [rf.PREC_FP64, rf.PREC_MIXED])
+def test_precond_apply_vs_numpy(precision):
+    N, K, M = 60, 4, 5
+    pos, ori = layout(N, 2e-2, 43)
+    with _sim(pos, ori, precond=1, precond_neighbours=K, precision=precision) as sim:
+        sim.assemble_system()
+        A = sim.matrix()
+        sim.solve_system()
+        nbr, _ = sim.neighbours(K)
+        r = np.random.default_rng(0).normal(size=sim.n)
+        z = sim.precond_apply(r)
+    m = 3 * M
+    zr = np.empty_like(r)
+    for i in range(N):
+        C = np.concatenate([[i], nbr[i]])
+        idx = np.concatenate([np.arange(c * m, (c + 1) * m) for c in C])
+        zr[i * m:(i + 1) * m] = np.linalg.solve(A[np.ix_(idx, idx)], r[idx])[:m]
+    assert np.abs(z - zr).max() <= 1e-10 * np.abs(zr).max()
+
+
+def test_precond_gmres_same_solution_fewer_iterations():
+    pos, ori = _hard_state()
+    out = {}
+    for pc in (0, 1):
+        with _sim(pos, ori, precond=pc, precond_neighbours=4) as sim:
+            sim.assemble_system()
+            sim.solve_system()
+            st = sim.stats()
+            out[pc] = (sim.coefficients(), st.last_gmres_iters, st.last_backward_error, st.last_solver)
+    x0, it0, be0, s0 = out[0]
+    x1, it1, be1, s1 = out[1]
+    assert s1 == rf.SOLVER_GMRES and be1 <= 1e-10
+    assert it1 * 5 <= it0, (it0, it1)
+    assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)
+
+
+def test_mixed_precision_matrix_and_solution():
+    N = 200
+    pos, ori = layout(N, 2e-2, 53)
+    res = {}
+    for prec in (rf.PREC_FP64, rf.PREC_MIXED):
+        with _sim(pos, ori, precision=prec, precond=1) as sim:
+            sim.assemble_system()
+            A, b = sim.matrix(), sim.rhs()
+            sim.solve_system()
+            x = sim.coefficients()
+            sim.update_velocities()
+            v, w = sim.velocities()
+            res[prec] = (A, b, x, v, w)
+    A64, b64, x64, v64, w64 = res[rf.PREC_FP64]
+    A32, b32, x32, v32, w32 = res[rf.PREC_MIXED]
+    scale = np.abs(A64 - np.eye(A64.shape[0])).max()
+    assert np.abs(A32 - A64).max() <= 1e-6 * scale
+    np.testing.assert_allclose(b32, b64, rtol=0, atol=1e-12 * np.abs(b64).max())  # rhs stays FP64
+    assert np.linalg.norm(x32 - x64) <= 1e-5 * np.linalg.norm(x64)
+    assert np.linalg.norm(v32 - v64) <= 1e-6 * np.linalg.norm(v64)
+
+
+def test_mixed_lu_with_refinement():
+    N = 120
+    pos, ori = layout(N, 2e-2, 59)
+    with _sim(pos, ori, use_direct_solver=1) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        x64 = sim.coefficients()
+    with _sim(pos, ori, use_direct_solver=1, precision=rf.PREC_MIXED, mixed_inner_tol=1e-12) as sim:
+        sim.assemble_system()
+        A32, b = sim.matrix(), sim.rhs()
+        sim.solve_system()
+        x32 = sim.coefficients()
+        st = sim.stats()
+    # refined against the FP64-arithmetic residual of the stored operator
+    assert st.last_backward_error <= 1e-12
+    assert np.linalg.norm(b - A32 @ x32) <= 1e-11 * np.linalg.norm(b)
+    assert np.linalg.norm(x32 - x64) <= 1e-5 * np.linalg.norm(x64)

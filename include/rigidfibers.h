@@ -48,7 +48,7 @@ extern "C" {
 #define RF_WARM_PREV 0    /* previous step's coefficients (fortran/advect_fibers.f90:464) */
 #define RF_WARM_ZERO 1    /* x0 = 0 every step */
 #define RF_WARM_EXTRAP 2  /* 2 x_n - x_{n-1} (linear extrapolation in time) */
-#define RF_WARM_LEARNED 3 /* x_{n} + MLP(now) - MLP(prev): learned increment */
+#define RF_WARM_LEARNED 3 /* min-residual combination of x_prev and the MLP prediction */
 
 /* GMRES preconditioners (the reference runs GMRES without one) */
 #define RF_PRECOND_NONE 0
@@ -115,7 +115,10 @@ typedef struct rf_stats {
     double matrix_bytes;            /* bytes of one system-matrix copy (n*lda*elem) */
     int32_t matrix_elem_bytes;      /* 8 FP64, 4 FP32 */
     int32_t lda;
-    double reserved[8];
+    double last_initial_residual;   /* ||b - A x0|| / ||b|| of the last GMRES solve */
+    double last_guess_alpha;        /* learned guess: x0 = alpha x_prev + beta MLP */
+    double last_guess_beta;
+    double reserved[5];
 } rf_stats;
 
 typedef struct rf_sim rf_sim;
@@ -127,7 +130,7 @@ void rf_params_default(rf_params *p);
 /* v1 parameter file (src/parameters.cc:124-180; fortran/advect_fibers.f90:182-222).
  * Lines after the last mandatory one may hold "key value" extensions:
  * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1,
- * precond none|neighbour, precond_neighbours K. */
+ * lu_block NB, precond none|neighbour, precond_neighbours K, mixed_inner_tol T. */
 int rf_parse_parameters(const char *path, rf_params *out);
 
 /* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per
@@ -167,13 +170,42 @@ int rf_get_stats(const rf_sim *sim, rf_stats *stats);
 int rf_set_coefficients(rf_sim *sim, const double *x);
 
 /* The k nearest fibers of every fiber by segment-segment distance at the
- * current configuration (ascending; nbr and dist hold N*k entries).  Needs
- * k <= the neighbours the handle keeps (precond_neighbours, or 16 with the
- * learned warm start). No counterpart in the reference. */
+ * current configuration (ascending, ties by index; nbr and dist hold N*k
+ * entries), 1 <= k <= min(16, N-1).  No counterpart in the reference. */
 int rf_get_neighbours(rf_sim *sim, int32_t k, int32_t *nbr, double *dist);
 /* z = M^-1 r with the neighbour preconditioner built by the last rf_solve
  * (n doubles each).  No counterpart in the reference. */
 int rf_precond_apply(rf_sim *sim, const double *r, double *z);
+
+/* Learned initial guess.  No counterpart in the reference (the reference
+ * warm-starts GMRES from the previous step only, fortran/advect_fibers.f90:464).
+ * A per-fiber network (FP32, 113 -> 128 -> 128 -> 128 -> 15, tanh) reads the
+ * relative centres, orientations and proximities of the fiber's 16 nearest
+ * fibers in the fiber's own frame (plus its tilt against gravity) and
+ * predicts its 3M = 15 coefficients (needs M = 5 and N >= 17).  With
+ * warm_start = RF_WARM_LEARNED and weights loaded, each GMRES solve starts
+ * from alpha x_prev + beta x_mlp, (alpha, beta) minimising the residual.
+ * Weight blob: rf_mlp_num_weights() floats, layout in_mean[113] in_std[113]
+ * W1[128][113] b1[128] W2[128][128] b2 W3[128][128] b3 W4[15][128] b4[15]
+ * out_mean[15] out_std[15]. */
+int64_t rf_mlp_num_weights(void);
+int rf_mlp_set_weights(rf_sim *sim, const float *w);
+int rf_mlp_get_weights(rf_sim *sim, float *w);
+/* Training pairs of the current configuration after rf_solve: X[N][113]
+ * features, Y[N][15] solved coefficients in each fiber's frame. */
+int rf_mlp_samples(rf_sim *sim, float *X, float *Y);
+/* Network prediction of the coefficient vector (n doubles, global frame). */
+int rf_mlp_predict(rf_sim *sim, double *x);
+/* Mean squared error (standardised targets) and its gradient (blob layout,
+ * zero in the standardisation entries) of the loaded network on B pairs. */
+int rf_mlp_forward_backward(rf_sim *sim, const float *X, const float *Y, int32_t B, float *grad, double *loss);
+/* Trains on `count` pairs on the GPU: minibatch Adam, cosine learning rate
+ * from lr, one shuffle per epoch from seed.  init = 1 starts from
+ * Glorot-uniform weights and takes the standardisation from the data;
+ * init = 0 continues from the loaded weights.  loss_per_epoch (epochs
+ * floats, may be NULL) receives the mean training loss of every epoch. */
+int rf_mlp_train(rf_sim *sim, const float *X, const float *Y, int64_t count, int32_t epochs, int32_t batch,
+                 float lr, uint64_t seed, int32_t init, float *loss_per_epoch);
 
 /* Waits for all work queued on the handle's stream. */
 int rf_synchronize(rf_sim *sim);

@@ -85,7 +85,10 @@ class rf_stats(ctypes.Structure):
         ("matrix_bytes", ctypes.c_double),
         ("matrix_elem_bytes", ctypes.c_int32),
         ("lda", ctypes.c_int32),
-        ("reserved", ctypes.c_double * 8),
+        ("last_initial_residual", ctypes.c_double),
+        ("last_guess_alpha", ctypes.c_double),
+        ("last_guess_beta", ctypes.c_double),
+        ("reserved", ctypes.c_double * 5),
     ]
 
     def as_dict(self):
@@ -93,7 +96,11 @@ class rf_stats(ctypes.Structure):
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _lib = None
+
+# learned initial guess: features per fiber, coefficients per fiber
+MLP_INPUTS, MLP_OUTPUTS, MLP_NEIGHBOURS = 113, 15, 16
 
 
 class RigidFibersError(RuntimeError):
@@ -130,6 +137,14 @@ _SIGNATURES = {
     "rf_get_neighbours": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]),
     "rf_precond_apply": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "rf_selftest_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _dp]),
+    "rf_mlp_num_weights": (ctypes.c_int64, []),
+    "rf_mlp_set_weights": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "rf_mlp_get_weights": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "rf_mlp_samples": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp]),
+    "rf_mlp_predict": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "rf_mlp_forward_backward": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int32, _fp, _dp]),
+    "rf_mlp_train": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_float, ctypes.c_uint64, ctypes.c_int32, _fp]),
     "rf_write_step_outputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]),
     "rf_selftest_mfma_layout": (ctypes.c_int, [ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_float)]),
     "rf_num_fibers": (ctypes.c_int32, [ctypes.c_void_p]),
@@ -175,6 +190,15 @@ def selftest_mfma_layout(mode):
     _check(lib().rf_selftest_mfma_layout(mode, _p(d), f.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
            "rf_selftest_mfma_layout")
     return d.reshape(64, 4), f.reshape(64, 4).astype(np.float64)
+
+
+def _f(a):
+    return a.ctypes.data_as(_fp)
+
+
+def mlp_num_weights():
+    """Floats in a weight blob of the learned-guess network."""
+    return int(lib().rf_mlp_num_weights())
 
 
 def selftest_copy(nbytes, reps=10):
@@ -335,6 +359,53 @@ class Simulation:
         z = np.empty(self.n)
         _check(lib().rf_precond_apply(self._h, _p(r), _p(z)), "rf_precond_apply")
         return z
+
+    # -- learned initial guess (no counterpart in the reference) -------------
+
+    def mlp_samples(self):
+        """(features [N,113], local-frame coefficients [N,15]) of the current
+        configuration and its solved coefficients (after solve_system)."""
+        X = np.empty((self.N, MLP_INPUTS), dtype=np.float32)
+        Y = np.empty((self.N, MLP_OUTPUTS), dtype=np.float32)
+        _check(lib().rf_mlp_samples(self._h, _f(X), _f(Y)), "rf_mlp_samples")
+        return X, Y
+
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != mlp_num_weights():
+            raise ValueError("weight blob must hold %d floats" % mlp_num_weights())
+        _check(lib().rf_mlp_set_weights(self._h, _f(w)), "rf_mlp_set_weights")
+
+    def mlp_weights(self):
+        w = np.empty(mlp_num_weights(), dtype=np.float32)
+        _check(lib().rf_mlp_get_weights(self._h, _f(w)), "rf_mlp_get_weights")
+        return w
+
+    def mlp_predict(self):
+        """Network prediction of the coefficient vector (n doubles, global frame)."""
+        x = np.empty(self.n)
+        _check(lib().rf_mlp_predict(self._h, _p(x)), "rf_mlp_predict")
+        return x
+
+    def mlp_forward_backward(self, X, Y):
+        """(loss, gradient blob) of the loaded network on one batch."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.empty(mlp_num_weights(), dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        _check(lib().rf_mlp_forward_backward(self._h, _f(X), _f(Y), X.shape[0], _f(g), ctypes.byref(loss)),
+               "rf_mlp_forward_backward")
+        return loss.value, g
+
+    def mlp_train(self, X, Y, epochs=30, batch=256, lr=1e-3, seed=0, init=True):
+        """Trains the network on the GPU (Adam, cosine schedule); returns the
+        mean training loss of every epoch."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        hist = np.empty(epochs, dtype=np.float32)
+        _check(lib().rf_mlp_train(self._h, _f(X), _f(Y), X.shape[0], epochs, batch, lr, seed, int(init), _f(hist)),
+               "rf_mlp_train")
+        return hist
 
     def synchronize(self):
         _check(lib().rf_synchronize(self._h), "rf_synchronize")

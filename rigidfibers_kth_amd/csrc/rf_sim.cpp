@@ -355,6 +355,7 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
     double beta2;
     if ((rc = norm2(d_r, &beta2))) return rc;
     double beta = std::sqrt(beta2);
+    stats.last_initial_residual = beta / bn;
     if (beta == 0.0) {
         *be_out = 0.0;
         return RF_OK;
@@ -535,7 +536,95 @@ int rf_sim::setup_neighbours()
         allocs.push_back(q);
         d_nbr_dist = static_cast<double*>(q);
     }
+    if (nbr_valid) return RF_OK;
     HIP_TRY(launch_knn(N, knn, d_pos[lvl_three], d_orient[lvl_three], d_nbr, d_nbr_dist, stream));
+    nbr_valid = true;
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned initial guess: features of the 16 nearest fibers -> MLP -> the
+// fiber's coefficients, combined with the previous solution by a
+// two-vector minimum-residual projection.
+// ---------------------------------------------------------------------------
+
+int rf_sim::mlp_alloc()
+{
+    if (M != kMlpTerms || knn < kMlpNeighbours) {
+        rf::set_error("learned guess: needs M = 5 and at least 17 fibers");
+        return RF_EINVAL;
+    }
+    if (d_mlp_w) return RF_OK;
+    const MlpLayout L = mlp_layout();
+    const int64_t floats = L.total + (int64_t)N * (kMlpIn + kMlpOut) + mlp_buffer_floats(N);
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(float) * floats) != hipSuccess) {
+        rf::set_error("learned guess: cannot allocate network buffers");
+        return RF_ENOMEM;
+    }
+    allocs.push_back(q);
+    d_mlp_w = static_cast<float*>(q);
+    d_mlp_X = d_mlp_w + L.total;
+    d_mlp_Y = d_mlp_X + (int64_t)N * kMlpIn;
+    d_mlp_work = d_mlp_Y + (int64_t)N * kMlpOut;
+    mlp_buffer_bind(d_mlp_work, N, &mlp_buf);
+    return RF_OK;
+}
+
+// x_out (device, n doubles) = network prediction at the current configuration.
+int rf_sim::mlp_infer(double* x_out)
+{
+    int rc = mlp_alloc();
+    if (rc) return rc;
+    if (!mlp_weights) {
+        rf::set_error("learned guess: no network weights loaded");
+        return RF_EINVAL;
+    }
+    if ((rc = setup_neighbours())) return rc;
+    HIP_TRY(mlp_features(N, knn, d_pos[lvl_three], d_orient[lvl_three], d_nbr, d_nbr_dist, d_mlp_X, stream));
+    HIP_TRY(mlp_forward(d_mlp_w, N, d_mlp_X, mlp_buf, stream));
+    HIP_TRY(mlp_to_global(N, M, d_orient[lvl_three], d_mlp_w, mlp_buf.Yhat, x_out, stream));
+    return RF_OK;
+}
+
+// d_x <- alpha d_xprev + beta g with g the prediction and (alpha, beta)
+// minimising ||b - A(alpha x_prev + beta g)||; costs two GEMVs.  The result
+// is never worse than the previous-step guess alone (alpha = 1, beta = 0 is
+// a candidate) up to rounding.
+int rf_sim::learned_guess()
+{
+    int rc = mlp_infer(d_guess);
+    if (rc) return rc;
+    const int64_t ldv = lda;
+    double* u = d_V;            // A x_prev
+    double* v = d_V + ldv;      // A g
+    double* bb = d_V + 2 * ldv; // b
+    if (restart < 2) {
+        rf::set_error("learned guess: needs gmres_restart >= 2");
+        return RF_EINVAL;
+    }
+    if ((rc = gemv_A(d_xprev, u, nullptr, 0))) return rc;
+    if ((rc = gemv_A(d_guess, v, nullptr, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(bb, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    // D[0..2] = (u.b, v.b, b.b), D[3..4] = (u.v, v.v), D[5] = u.u
+    HIP_TRY(launch_multidot(d_V, ldv, 2, bb, n, d_dparts, d_counter, d_dots, stream));
+    HIP_TRY(launch_multidot(d_V, ldv, 1, v, n, d_dparts, d_counter, d_dots + 3, stream));
+    HIP_TRY(launch_multidot(nullptr, 0, 0, u, n, d_dparts, d_counter, d_dots + 5, stream));
+    if ((rc = fetch_dots(6))) return rc;
+    const double ub = h_dots[0], vb = h_dots[1], uv = h_dots[3], vv = h_dots[4], uu = h_dots[5];
+    double alpha = 0.0, beta = 0.0;
+    const double det = uu * vv - uv * uv;
+    if (uu > 0.0 && vv > 0.0 && det > 1e-12 * uu * vv) {
+        alpha = (ub * vv - vb * uv) / det;
+        beta = (vb * uu - ub * uv) / det;
+    } else if (vv > 0.0 && (uu == 0.0 || vb * vb / vv > ub * ub / uu)) {
+        beta = vb / vv;
+    } else if (uu > 0.0) {
+        alpha = ub / uu;
+    }
+    stats.last_guess_alpha = alpha;
+    stats.last_guess_beta = beta;
+    HIP_TRY(launch_axpby(n, alpha, d_xprev, beta, d_guess, d_x, stream));
     return RF_OK;
 }
 
@@ -659,6 +748,8 @@ int rf_sim::solve()
         HIP_TRY(hipMemcpyAsync(d_x, d_guess, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
     } else {
         HIP_TRY(hipMemcpyAsync(d_xprev, d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        if (p.warm_start == RF_WARM_LEARNED && mlp_weights && !p.use_direct_solver && (rc = learned_guess()))
+            return rc;
     }
     int iters = 0, status = 0;
     double be = 0.0;
@@ -735,6 +826,7 @@ int rf_sim::advance()
     stats.steps_taken = steps_taken;
     assembled = solved = have_velocities = false;
     prec_ready = false;
+    nbr_valid = false;
     return RF_OK;
 }
 
@@ -786,7 +878,9 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     s->mixed = (P.precision == RF_PREC_MIXED) ? 1 : 0;
     s->restart = P.gmres_restart;
     s->ras_k = P.precond ? std::min(num_fibers - 1, P.precond_neighbours > 0 ? P.precond_neighbours : 4) : 0;
-    s->knn = s->ras_k;
+    // neighbour lists keep the 16 nearest fibers (the network's input); the
+    // preconditioner uses the first ras_k of them
+    s->knn = std::min(num_fibers - 1, kMlpNeighbours);
     build_tables(s->M, P.num_quad_intervals, P.slenderness, &s->tab);
     s->LQ = s->tab.LQ;
     s->lda = round_up(s->n, 256);
@@ -1032,6 +1126,152 @@ extern "C" int rf_precond_apply(rf_sim* s, const double* r, double* z)
     int rc = s->precond_apply(s->d_t, s->d_z);
     if (rc) return rc;
     return copy_out(s, z, s->d_z, (size_t)s->n);
+}
+
+// ---- learned initial guess ------------------------------------------------
+
+extern "C" int64_t rf_mlp_num_weights(void) { return mlp_layout().total; }
+
+extern "C" int rf_mlp_set_weights(rf_sim* s, const float* w)
+{
+    CHECK_SIM(s);
+    if (!w) {
+        set_error("rf_mlp_set_weights: null");
+        return RF_EINVAL;
+    }
+    int rc = s->mlp_alloc();
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(s->d_mlp_w, w, sizeof(float) * mlp_layout().total, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->mlp_weights = true;
+    return RF_OK;
+}
+
+extern "C" int rf_mlp_get_weights(rf_sim* s, float* w)
+{
+    CHECK_SIM(s);
+    if (!w || !s->mlp_weights) {
+        set_error("rf_mlp_get_weights: no weights loaded");
+        return RF_EINVAL;
+    }
+    HIP_TRY(hipMemcpyAsync(w, s->d_mlp_w, sizeof(float) * mlp_layout().total, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return RF_OK;
+}
+
+extern "C" int rf_mlp_samples(rf_sim* s, float* X, float* Y)
+{
+    CHECK_SIM(s);
+    if (!X || !Y || !s->solved) {
+        set_error("rf_mlp_samples: needs output buffers and a solved system (rf_solve) at this configuration");
+        return RF_EINVAL;
+    }
+    int rc = s->mlp_alloc();
+    if (rc) return rc;
+    if ((rc = s->setup_neighbours())) return rc;
+    HIP_TRY(mlp_features(s->N, s->knn, s->d_pos[s->lvl_three], s->d_orient[s->lvl_three], s->d_nbr, s->d_nbr_dist,
+                         s->d_mlp_X, s->stream));
+    HIP_TRY(mlp_targets(s->N, s->M, s->d_orient[s->lvl_three], s->d_x, s->d_mlp_Y, s->stream));
+    HIP_TRY(hipMemcpyAsync(X, s->d_mlp_X, sizeof(float) * (size_t)s->N * kMlpIn, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(Y, s->d_mlp_Y, sizeof(float) * (size_t)s->N * kMlpOut, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return RF_OK;
+}
+
+extern "C" int rf_mlp_predict(rf_sim* s, double* x)
+{
+    CHECK_SIM(s);
+    if (!x) {
+        set_error("rf_mlp_predict: null");
+        return RF_EINVAL;
+    }
+    int rc = s->mlp_infer(s->d_guess);
+    if (rc) return rc;
+    return copy_out(s, x, s->d_guess, (size_t)s->n);
+}
+
+extern "C" int rf_mlp_forward_backward(rf_sim* s, const float* X, const float* Y, int32_t B, float* grad,
+                                       double* loss)
+{
+    CHECK_SIM(s);
+    if (!X || !Y || !grad || !loss || B < 1 || !s->mlp_weights) {
+        set_error("rf_mlp_forward_backward: bad arguments or no weights loaded");
+        return RF_EINVAL;
+    }
+    const MlpLayout L = mlp_layout();
+    const int64_t used = L.total + (int64_t)B * (kMlpIn + kMlpOut) + mlp_buffer_floats(B);
+    const int64_t floats = used + 4;  // + one 8-byte aligned double after `used`
+    float* q = nullptr;
+    if (hipMalloc(&q, sizeof(float) * floats) != hipSuccess) {
+        set_error("rf_mlp_forward_backward: cannot allocate");
+        return RF_ENOMEM;
+    }
+    float* g = q;
+    float* dX = g + L.total;
+    float* dY = dX + (int64_t)B * kMlpIn;
+    float* work = dY + (int64_t)B * kMlpOut;
+    double* acc = reinterpret_cast<double*>(q + (used + 1) / 2 * 2);
+    MlpBuffers buf;
+    mlp_buffer_bind(work, B, &buf);
+    hipError_t e = hipMemcpyAsync(dX, X, sizeof(float) * (size_t)B * kMlpIn, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dY, Y, sizeof(float) * (size_t)B * kMlpOut, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(g, 0, sizeof(float) * L.total, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(acc, 0, sizeof(double), s->stream);
+    if (e == hipSuccess) e = mlp_forward(s->d_mlp_w, B, dX, buf, s->stream);
+    if (e == hipSuccess) e = mlp_loss(s->d_mlp_w, B, dY, buf, s->stream);
+    if (e == hipSuccess) e = mlp_accumulate_loss(B, buf, acc, s->stream);
+    if (e == hipSuccess) e = mlp_backward(s->d_mlp_w, B, buf, g, s->stream);
+    double a = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(grad, g, sizeof(float) * L.total, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&a, acc, sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = e2;
+    (void)hipFree(q);
+    if (e != hipSuccess) {
+        set_error(std::string("rf_mlp_forward_backward: ") + hipGetErrorString(e));
+        return RF_EHIP;
+    }
+    *loss = a / ((double)B * kMlpOut);
+    return RF_OK;
+}
+
+extern "C" int rf_mlp_train(rf_sim* s, const float* X, const float* Y, int64_t count, int32_t epochs,
+                            int32_t batch, float lr, uint64_t seed, int32_t init, float* loss_per_epoch)
+{
+    CHECK_SIM(s);
+    if (!X || !Y || count < 1 || epochs < 1 || batch < 1 || !(lr > 0.f) || (!init && !s->mlp_weights)) {
+        set_error("rf_mlp_train: bad arguments (or init = 0 without loaded weights)");
+        return RF_EINVAL;
+    }
+    int rc = s->mlp_alloc();
+    if (rc) return rc;
+    const MlpLayout L = mlp_layout();
+    if (init) {
+        std::vector<float> w(L.total, 0.f);
+        mlp_init_weights(seed, w.data());
+        mlp_set_standardisation(X, Y, count, w.data());
+        HIP_TRY(hipMemcpyAsync(s->d_mlp_w, w.data(), sizeof(float) * L.total, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+    float* q = nullptr;
+    if (hipMalloc(&q, sizeof(float) * count * (kMlpIn + kMlpOut)) != hipSuccess) {
+        set_error("rf_mlp_train: cannot allocate the training set on the device");
+        return RF_ENOMEM;
+    }
+    hipError_t e = hipMemcpyAsync(q, X, sizeof(float) * count * kMlpIn, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(q + count * kMlpIn, Y, sizeof(float) * count * kMlpOut, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = mlp_train(s->d_mlp_w, q, q + count * kMlpIn, count, epochs, batch, lr, seed, loss_per_epoch, s->stream);
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess) e = e2;
+    (void)hipFree(q);
+    if (e != hipSuccess) {
+        set_error(std::string("rf_mlp_train: ") + hipGetErrorString(e));
+        return RF_EHIP;
+    }
+    s->mlp_weights = true;
+    return RF_OK;
 }
 
 extern "C" int rf_synchronize(rf_sim* s)

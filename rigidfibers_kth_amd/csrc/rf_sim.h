@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "rf_internal.h"
+#include "rf_mlp.h"
 
 struct rf_sim {
     rf_params p{};
@@ -69,6 +70,14 @@ struct rf_sim {
     double* d_t = nullptr;
     int* d_pinfo = nullptr;
     bool prec_ready = false;
+    bool nbr_valid = false;  // neighbour lists match the current configuration
+    // learned initial guess (allocated on first use)
+    float* d_mlp_w = nullptr;    // weight blob
+    float* d_mlp_X = nullptr;    // [N][113] features
+    float* d_mlp_Y = nullptr;    // [N][15] local-frame targets
+    float* d_mlp_work = nullptr; // activations for N rows
+    rf::MlpBuffers mlp_buf;
+    bool mlp_weights = false;    // a trained network is loaded
     // LU workspace (allocated on first use)
     void* d_LU = nullptr;
     int* d_ipiv = nullptr;
@@ -102,6 +111,9 @@ struct rf_sim {
     int solve_gmres(int* iters, double* be, int* status);
     int solve_lu(double* be);
     int setup_neighbours();
+    int mlp_alloc();
+    int mlp_infer(double* x_out);
+    int learned_guess();
     int setup_precond();
     int precond_apply(const double* r, double* z);
     int update_iterate(int k);

@@ -4,8 +4,9 @@
 * the neighbour-coupled block-Jacobi preconditioner vs a numpy restatement
   (cluster = fiber + K nearest, rows of the cluster inverse), and GMRES with it
   reaching the same solution as the reference's unpreconditioned GMRES;
-* mixed precision (FP32 matrix with FP64 near field, FP64 Arnoldi and
-  residuals): error against the FP64 path, FP32 LU with FP64 refinement.
+* mixed precision (FP32 matrix with FP64 near field and FP64 closed forms,
+  FP64 Arnoldi and residuals): error against the FP64 path, FP32 LU with
+  FP64 refinement.
 """
 import numpy as np
 import pytest
@@ -100,7 +101,9 @@ def test_mixed_precision_matrix_and_solution():
     A32, b32, x32, v32, w32 = res[rf.PREC_MIXED]
     scale = np.abs(A64 - np.eye(A64.shape[0])).max()
     assert np.abs(A32 - A64).max() <= 1e-6 * scale
-    np.testing.assert_allclose(b32, b64, rtol=0, atol=1e-12 * np.abs(b64).max())  # rhs stays FP64
+    # the rhs shares the matrix's far-field integrals (FP32 backward
+    # recurrences, FP64 closed forms): ~5e-7 of its scale
+    assert np.abs(b32 - b64).max() <= 2e-6 * np.abs(b64).max()
     assert np.linalg.norm(x32 - x64) <= 1e-5 * np.linalg.norm(x64)
     assert np.linalg.norm(v32 - v64) <= 1e-6 * np.linalg.norm(v64)
 

@@ -53,6 +53,7 @@ extern "C" {
 /* GMRES preconditioners (the reference runs GMRES without one) */
 #define RF_PRECOND_NONE 0
 #define RF_PRECOND_NEIGHBOUR 1 /* block-Jacobi on each fiber + its K nearest fibers (restricted additive Schwarz) */
+#define RF_PRECOND_LU 2        /* LU factors of the stored matrix (FP32 in the mixed mode: GMRES-IR) */
 
 /* output number formats */
 #define RF_FMT_F16_8 0  /* '(*(F16.8))' as the Fortran's *.out / tests/reference *.ref */
@@ -130,7 +131,7 @@ void rf_params_default(rf_params *p);
 /* v1 parameter file (src/parameters.cc:124-180; fortran/advect_fibers.f90:182-222).
  * Lines after the last mandatory one may hold "key value" extensions:
  * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1,
- * lu_block NB, precond none|neighbour, precond_neighbours K, mixed_inner_tol T. */
+ * lu_block NB, precond none|neighbour|lu, precond_neighbours K, mixed_inner_tol T. */
 int rf_parse_parameters(const char *path, rf_params *out);
 
 /* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per

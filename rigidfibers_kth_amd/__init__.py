@@ -26,7 +26,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 WARM_PREV, WARM_ZERO, WARM_EXTRAP, WARM_LEARNED = 0, 1, 2, 3
 FMT_F16_8, FMT_F24_16 = 0, 1
 SOLVER_GMRES, SOLVER_LU, SOLVER_LU_FALLBACK = 0, 1, 2
-PRECOND_NONE, PRECOND_NEIGHBOUR = 0, 1
+PRECOND_NONE, PRECOND_NEIGHBOUR, PRECOND_LU = 0, 1, 2
 
 
 class rf_params(ctypes.Structure):

@@ -161,6 +161,7 @@ extern "C" int rf_parse_parameters(const char* path, rf_params* out)
         } else if (key == "precond") {
             if (val == "none") p.precond = RF_PRECOND_NONE;
             else if (val == "neighbour") p.precond = RF_PRECOND_NEIGHBOUR;
+            else if (val == "lu") p.precond = RF_PRECOND_LU;
             else ok = false;
         } else if (key == "precond_neighbours") {
             ok = parse_int(val, &p.precond_neighbours);

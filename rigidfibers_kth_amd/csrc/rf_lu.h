@@ -9,9 +9,12 @@ namespace rf {
 
 // In-place LU of the n x n column-major matrix (double if elem_bytes == 8,
 // float otherwise); ipiv[k] = row swapped with k (0-based); *info = k+1 if
-// U(k,k) == 0.  use_mfma selects the matrix-core trailing update.
+// U(k,k) == 0.  use_mfma selects the matrix-core trailing update.  ws:
+// lu_workspace_bytes() of device memory whose last word (the column
+// kernels' counter) is zero on entry and is left zero.
+size_t lu_workspace_bytes();
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     hipStream_t s);
+                     void* ws, hipStream_t s);
 // x <- U^-1 L^-1 P x with FP64 vectors.
 hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s);
 

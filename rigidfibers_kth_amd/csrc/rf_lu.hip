@@ -16,6 +16,7 @@
 #include "rf_lu.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace rf {
 namespace {
@@ -23,28 +24,13 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kPivThreads = 1024;
+constexpr int kColThreads = 256;
+constexpr int kColMaxBlocks = 512;
 
-// Column k: pivot search over rows [k, n), swap rows k and p over columns
-// [c0, c1), scale the subdiagonal of column k.  One workgroup.
-template <typename T>
-__global__ void __launch_bounds__(kPivThreads) lu_pivot_kernel(T* A, int64_t lda, int n, int k, int c0, int c1,
-                                                               int* ipiv, int* info)
+// (value, index) maximum with ties to the smaller index (LAPACK idamax keeps
+// the first maximal entry) over one workgroup; result valid in thread 0.
+__device__ __forceinline__ void block_argmax(double& best, int& bi, double* sv, int* si)
 {
-    __shared__ double sv[kPivThreads / 64];
-    __shared__ int si[kPivThreads / 64];
-    __shared__ int piv_s;
-    const T* col = A + (int64_t)k * lda;
-    double best = -1.0;
-    int bi = n;
-    for (int i = k + threadIdx.x; i < n; i += kPivThreads) {
-        const double v = fabs((double)col[i]);
-        if (v > best) {  // strict: keeps the first index among equals in this thread
-            best = v;
-            bi = i;
-        }
-    }
-    // wave reduction: max value, then smallest index
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const double ov = __shfl_xor(best, off, 64);
@@ -61,49 +47,154 @@ __global__ void __launch_bounds__(kPivThreads) lu_pivot_kernel(T* A, int64_t lda
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double b = sv[0];
-        int idx = si[0];
-        for (int q = 1; q < kPivThreads / 64; ++q)
-            if (sv[q] > b || (sv[q] == b && si[q] < idx)) {
-                b = sv[q];
-                idx = si[q];
+        for (int q = 1; q < (int)(blockDim.x / 64); ++q)
+            if (sv[q] > best || (sv[q] == best && si[q] < bi)) {
+                best = sv[q];
+                bi = si[q];
             }
-        if (idx >= n) idx = k;
-        piv_s = idx;
-        ipiv[k] = idx;
+    }
+}
+
+// One column of the left-looking (Crout) panel factorisation; one launch per
+// column j of the panel [k0, k1).  On entry the panel columns < j hold L
+// (column j-1 not yet divided by its pivot), rows [k0, j) of column j hold
+// U[k0..j, j], and every earlier row swap of the panel has been applied to
+// all panel columns.  The launch
+//  * divides column j-1 by its pivot (rows >= j),
+//  * forms v_i = A[i,j] - sum_{q in [k0,j)} L[i,q] U[q,j] for rows i >= j
+//    and stores it in column j,
+//  * and the last workgroup to finish (atomic counter) picks the pivot of
+//    column j (first largest |v_i|, LAPACK idamax), records ipiv[j], swaps
+//    rows j and p over the panel columns, and computes U[k0..j+1, j+1] of the
+//    next column by forward substitution with the unit lower L11.
+// Only two columns are written per launch (the right-looking form rewrites
+// the whole rest of the panel every column).
+template <typename T>
+__global__ void __launch_bounds__(kColThreads) lu_crout_kernel(T* A, int64_t lda, int n, int k0, int k1, int j,
+                                                               int* ipiv, int* info, double* pval, int* pidx,
+                                                               unsigned int* counter)
+{
+    __shared__ double sU[64];
+    __shared__ double sL[64][65];  // L11 rows for the next column's U
+    __shared__ double sB[64];
+    __shared__ double sv[kColThreads / 64];
+    __shared__ int si[kColThreads / 64];
+    __shared__ int last_s, piv_s;
+    const int m = j - k0;  // columns of L to the left of j inside the panel
+    T* cj = A + (int64_t)j * lda;
+    for (int q = threadIdx.x; q < m; q += kColThreads) sU[q] = (double)cj[k0 + q];
+    double inv = 0.0;
+    if (m > 0) {
+        const double piv = (double)A[(int64_t)(j - 1) * lda + (j - 1)];
+        inv = (piv != 0.0) ? 1.0 / piv : 0.0;  // zero pivot: info is already set
+    }
+    __syncthreads();
+    double best = -1.0;
+    int bi = n;
+    for (int i = j + blockIdx.x * kColThreads + threadIdx.x; i < n; i += gridDim.x * kColThreads) {
+        double v = (double)cj[i];
+        if (m > 0) {
+            T* cp = A + (int64_t)(j - 1) * lda;
+            const double l = (double)cp[i] * inv;
+            cp[i] = (T)l;
+            const T* li = A + (int64_t)k0 * lda + i;
+            int q = 0;
+            for (; q + 8 <= m - 1; q += 8) {  // eight loads in flight
+                double a[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] = (double)li[(int64_t)(q + u) * lda];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v -= a[u] * sU[q + u];
+            }
+            for (; q < m - 1; ++q) v -= (double)li[(int64_t)q * lda] * sU[q];
+            v -= l * sU[m - 1];
+        }
+        const T vt = (T)v;
+        cj[i] = vt;
+        const double a = fabs((double)vt);
+        if (a > best) {  // strict: first index among equals in this thread
+            best = a;
+            bi = i;
+        }
+    }
+    block_argmax(best, bi, sv, si);
+    if (threadIdx.x == 0) {
+        pval[blockIdx.x] = best;
+        pidx[blockIdx.x] = bi;
+    }
+    __syncthreads();  // the workgroup's matrix and partial stores are issued
+    if (threadIdx.x == 0) {
+        __threadfence();  // ... and made visible device-wide before the count
+        last_s = (atomicAdd(counter, 1u) == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!last_s) return;
+    __threadfence();
+    best = -1.0;
+    bi = n;
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += kColThreads) {
+        const double v = __hip_atomic_load(&pval[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int ii = __hip_atomic_load(&pidx[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v > best || (v == best && ii < bi)) {
+            best = v;
+            bi = ii;
+        }
+    }
+    block_argmax(best, bi, sv, si);
+    if (threadIdx.x == 0) {
+        const int p = (bi < n) ? bi : j;
+        piv_s = p;
+        ipiv[j] = p;
+        if (!(best > 0.0) && *info == 0) *info = j + 1;
+        atomicExch(counter, 0u);
     }
     __syncthreads();
     const int p = piv_s;
-    if (p != k) {
-        for (int c = c0 + threadIdx.x; c < c1; c += kPivThreads) {
+    if (p != j) {
+        for (int c = k0 + threadIdx.x; c < k1; c += kColThreads) {
             T* cc = A + (int64_t)c * lda;
-            const T t = cc[k];
-            cc[k] = cc[p];
+            const T t = cc[j];
+            cc[j] = cc[p];
             cc[p] = t;
         }
     }
     __syncthreads();
-    T* colw = A + (int64_t)k * lda;
-    const double akk = (double)colw[k];
-    if (akk == 0.0) {
-        if (threadIdx.x == 0 && *info == 0) *info = k + 1;
-        return;
+    // U[k0..j+1, j+1]: r = k0..j, U[r] = A[r, j+1] - sum_{q < r} L[r, q] U[q].
+    // L11 rows and the right-hand side are read into LDS with all threads at
+    // once; one wavefront then runs the substitution (lane q keeps U[k0 + q]).
+    if (j + 1 >= k1) return;
+    T* cn = A + (int64_t)(j + 1) * lda;
+    for (int e = threadIdx.x; e < (m + 1) * m; e += kColThreads) {
+        const int r = e % (m + 1), q = e / (m + 1);
+        sL[r][q] = (double)A[(int64_t)(k0 + q) * lda + k0 + r];
     }
-    const double r = 1.0 / akk;
-    for (int i = k + 1 + threadIdx.x; i < n; i += kPivThreads) colw[i] = (T)((double)colw[i] * r);
+    for (int r = threadIdx.x; r <= m; r += kColThreads) sB[r] = (double)cn[k0 + r];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        double u = 0.0;
+        for (int r = 0; r <= m; ++r) {
+            double t = (lane < r) ? sL[r][lane] * u : 0.0;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+            const double ur = sB[r] - t;
+            if (lane == r) u = ur;
+        }
+        if (lane <= m) cn[k0 + lane] = (T)u;
+    }
 }
 
-// A[i, c] -= A[i, k] * A[k, c] for i in (k, n), c in (k, c1)
+// Column c of L below the diagonal divided by the pivot (the panel's last
+// column, which no later Crout launch scales).
 template <typename T>
-__global__ void lu_panel_update_kernel(T* A, int64_t lda, int n, int k, int c1)
+__global__ void lu_scale_kernel(T* A, int64_t lda, int n, int c)
 {
-    const int i = k + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = c + 1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double lik = (double)A[(int64_t)k * lda + i];
-    for (int c = k + 1; c < c1; ++c) {
-        T* cc = A + (int64_t)c * lda;
-        cc[i] = (T)((double)cc[i] - lik * (double)cc[k]);
-    }
+    const double piv = (double)A[(int64_t)c * lda + c];
+    const double inv = (piv != 0.0) ? 1.0 / piv : 0.0;
+    T* cc = A + (int64_t)c * lda;
+    cc[i] = (T)((double)cc[i] * inv);
 }
 
 // Row swaps ipiv[k0..k1) applied to columns outside [k0, k1) (LAPACK dlaswp).
@@ -125,39 +216,46 @@ __global__ void lu_laswp_kernel(T* A, int64_t lda, int n, int k0, int k1, const 
     }
 }
 
-// U12 = L11^-1 A12: columns c in [k1, n), rows [k0, k1).  One thread per
-// column; L11 staged in LDS.
+// U12 = L11^-1 A12: columns c in [k1, n), rows [k0, k1).  A workgroup
+// moves a 128-column slab into LDS with column-contiguous (coalesced) global
+// accesses; each thread then runs the forward substitution of its column in
+// LDS against L11 (also in LDS).
+constexpr int kTrsmCols = 128;
+
 template <typename T, int NB>
-__global__ void __launch_bounds__(256) lu_trsm_kernel(T* A, int64_t lda, int n, int k0, int k1)
+__global__ void __launch_bounds__(kTrsmCols) lu_trsm_kernel(T* A, int64_t lda, int n, int k0, int k1)
 {
     __shared__ double L[NB][NB + 1];
+    __shared__ double S[NB][kTrsmCols + 1];
     const int nb = k1 - k0;
-    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+    for (int e = threadIdx.x; e < nb * nb; e += kTrsmCols) {
         const int r = e % nb, c = e / nb;
         L[r][c] = (double)A[(int64_t)(k0 + c) * lda + k0 + r];
     }
-    __syncthreads();
-    const int c = k1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    T* cc = A + (int64_t)c * lda + k0;
-    double x[NB];
-#pragma unroll
-    for (int r = 0; r < NB; ++r) x[r] = (r < nb) ? (double)cc[r] : 0.0;
-#pragma unroll
-    for (int r = 0; r < NB; ++r) {
-        const double xr = x[r];
-#pragma unroll
-        for (int q = r + 1; q < NB; ++q)
-            if (r < nb && q < nb) x[q] -= L[q][r] * xr;
+    const int cbase = k1 + blockIdx.x * kTrsmCols;
+    for (int e = threadIdx.x; e < nb * kTrsmCols; e += kTrsmCols) {
+        const int r = e % nb, c = e / nb;
+        const int col = cbase + c;
+        S[r][c] = (col < n) ? (double)A[(int64_t)col * lda + k0 + r] : 0.0;
     }
-#pragma unroll
-    for (int r = 0; r < NB; ++r)
-        if (r < nb) cc[r] = (T)x[r];
+    __syncthreads();
+    const int c = threadIdx.x;
+    for (int r = 0; r < nb; ++r) {
+        const double xr = S[r][c];
+        for (int q = r + 1; q < nb; ++q) S[q][c] -= L[q][r] * xr;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nb * kTrsmCols; e += kTrsmCols) {
+        const int r = e % nb, cc = e / nb;
+        const int col = cbase + cc;
+        if (col < n) A[(int64_t)col * lda + k0 + r] = (T)S[r][cc];
+    }
 }
 
 // ---- trailing update C -= L21 * U12 ------------------------------------
-// Workgroup tile 64x64, K staged 16 at a time in LDS.
 
+// VALU version (reference for the matrix-core kernels; any precision):
+// workgroup tile 64x64, K staged 16 at a time in LDS.
 constexpr int kTile = 64;
 constexpr int kKc = 16;
 
@@ -178,7 +276,6 @@ __device__ __forceinline__ void stage_tiles(const T* A, int64_t lda, int n, int 
     }
 }
 
-// VALU version (reference for the matrix-core kernels; any precision).
 template <typename T>
 __global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, int n, int k0, int k1)
 {
@@ -218,99 +315,164 @@ __global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, in
         }
 }
 
-// Matrix-core version: each of the 4 wavefronts owns a 32x32 quarter of the
-// 64x64 tile = 2x2 blocks of 16x16.  For v_mfma_*_16x16x4 lane l holds
-// A[l%16][l/16] and B[l/16][l%16]; accumulator r of lane l is
-// D[l/16 + 4r][l%16] for the f64 instruction and D[4(l/16) + r][l%16] for
-// the f32 one (measured by rf_selftest_mfma_layout).
-__global__ void __launch_bounds__(256) lu_gemm_mfma_f64_kernel(double* A, int64_t lda, int n, int k0, int k1)
+// Matrix-core version: workgroup tile 128 x BN (BN = 128 in FP32, 64 in
+// FP64), 4 wavefronts of 64 x BN/2 (blocks of 16x16), K staged 32 at a time
+// in LDS in the matrix's precision.
+// For v_mfma_*_16x16x4 lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// accumulator r of lane l is D[4(l/16) + r][l%16] for the f32 instruction
+// and D[l/16 + 4r][l%16] for the f64 one (rf_selftest_mfma_layout).  The
+// f32 product is D = L21 U12, so a lane's 4 values are 4 consecutive rows
+// of one column (one 16-byte access); the f64 product is taken transposed,
+// D = (L21 U12)^T, so that for each r a wavefront touches 16 consecutive
+// rows of 4 columns.
+constexpr int kBM = 128, kKC = 32, kLdsPad = 4;
+// columns per workgroup tile: 128 in FP32; 64 in FP64 keeps the accumulators
+// and the prefetch registers of a wavefront within two waves per SIMD
+template <typename T>
+constexpr int gemm_bn() { return sizeof(T) == 8 ? 64 : 128; }
+
+__device__ __forceinline__ void mfma_16x16x4(double a, double b, d4& acc)
 {
-    __shared__ double As[kTile][kKc + 1];
-    __shared__ double Bs[kKc][kTile + 1];
-    const int rbase = k1 + blockIdx.x * kTile, cbase = k1 + blockIdx.y * kTile;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    d4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int kc = k0; kc < k1; kc += kKc) {
-        const int kcount = min(kKc, k1 - kc);
-        stage_tiles<double>(A, lda, n, rbase, cbase, k0, kc, kcount, As, Bs);
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < kKc; ks += 4) {
-            double a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[wr + i * 16 + (lane & 15)][ks + (lane >> 4)];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[ks + (lane >> 4)][wc + j * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = rbase + wr + i * 16 + (lane >> 4) + 4 * r;
-                const int col = cbase + wc + j * 16 + (lane & 15);
-                if (row < n && col < n) {
-                    double* p = A + (int64_t)col * lda + row;
-                    *p -= acc[i][j][r];
-                }
-            }
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ void mfma_16x16x4(float a, float b, f4& acc)
+{
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
 }
 
-__global__ void __launch_bounds__(256) lu_gemm_mfma_f32_kernel(float* A, int64_t lda, int n, int k0, int k1)
-{
-    __shared__ double As[kTile][kKc + 1];
-    __shared__ double Bs[kKc][kTile + 1];
-    const int rbase = k1 + blockIdx.x * kTile, cbase = k1 + blockIdx.y * kTile;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    f4 acc[2][2];
+// Staging of one K chunk: As[kk][r] = L21[rbase + r, kc + kk],
+// Bs[kk][c] = U12[kc + kk, cbase + c].  With VEC the global reads are 16-byte
+// vectors along the contiguous dimension (rows of L21, K of U12) and are
+// issued into registers one chunk ahead of the matrix-core work.
+template <typename T>
+struct GemmStage {
+    static constexpr int VL = 16 / (int)sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VL)));
+    static constexpr int NA = kKC * kBM / VL / 256;  // vectors per thread
+    static constexpr int BN = gemm_bn<T>();
+    static constexpr int NB = kKC * BN / VL / 256;
+    vec_t a[NA], b[NB];
+
+    __device__ __forceinline__ void load(const T* A, int64_t lda, int n, int rbase, int cbase, int kc, int kcount,
+                                         bool vec)
+    {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+        for (int t = 0; t < NA; ++t) {
+            const int v = threadIdx.x + 256 * t;
+            const int kk = v / (kBM / VL), r0 = (v % (kBM / VL)) * VL;
+            const int row = rbase + r0;
+            const T* src = A + (int64_t)(kc + kk) * lda + row;
+            if (vec && kk < kcount && row + VL <= n) {
+                a[t] = *reinterpret_cast<const vec_t*>(src);
+            } else {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    for (int kc = k0; kc < k1; kc += kKc) {
-        const int kcount = min(kKc, k1 - kc);
-        stage_tiles<float>(A, lda, n, rbase, cbase, k0, kc, kcount, As, Bs);
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < kKc; ks += 4) {
-            float a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = (float)As[wr + i * 16 + (lane & 15)][ks + (lane >> 4)];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = (float)Bs[ks + (lane >> 4)][wc + j * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int u = 0; u < VL; ++u) a[t][u] = (kk < kcount && row + u < n) ? src[u] : T(0);
+            }
         }
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            const int v = threadIdx.x + 256 * t;
+            const int c = v / (kKC / VL), kk0 = (v % (kKC / VL)) * VL;
+            const int col = cbase + c;
+            const T* src = A + (int64_t)col * lda + kc + kk0;
+            if (vec && col < n && kk0 + VL <= kcount) {
+                b[t] = *reinterpret_cast<const vec_t*>(src);
+            } else {
+#pragma unroll
+                for (int u = 0; u < VL; ++u) b[t][u] = (col < n && kk0 + u < kcount) ? src[u] : T(0);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(T (&As)[kKC][kBM + kLdsPad], T (&Bs)[kKC][BN + kLdsPad]) const
+    {
+#pragma unroll
+        for (int t = 0; t < NA; ++t) {
+            const int v = threadIdx.x + 256 * t;
+            const int kk = v / (kBM / VL), r0 = (v % (kBM / VL)) * VL;
+            *reinterpret_cast<vec_t*>(&As[kk][r0]) = a[t];
+        }
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            const int v = threadIdx.x + 256 * t;
+            const int c = v / (kKC / VL), kk0 = (v % (kKC / VL)) * VL;
+#pragma unroll
+            for (int u = 0; u < VL; ++u) Bs[kk0 + u][c] = b[t][u];
+        }
+    }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, int n, int k0, int k1)
+{
+    using V4 = typename std::conditional<sizeof(T) == 8, d4, f4>::type;
+    constexpr bool kF64 = sizeof(T) == 8;
+    constexpr int BN = gemm_bn<T>(), JB = BN / 32;  // JB 16-column blocks per wavefront
+    __shared__ __attribute__((aligned(16))) T As[kKC][kBM + kLdsPad];
+    __shared__ __attribute__((aligned(16))) T Bs[kKC][BN + kLdsPad];
+    const int rbase = k1 + blockIdx.x * kBM, cbase = k1 + blockIdx.y * BN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave >> 1) * 64, wc = (wave & 1) * (BN / 2);
+    const int lr = lane & 15, lk = lane >> 4;
+    V4 acc[4][JB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) acc[i][j] = V4{0, 0, 0, 0};
+    GemmStage<T> st;
+    // 16-byte vectors need rows and K offsets that are multiples of the vector
+    const bool vec = (k0 % GemmStage<T>::VL == 0) && (k1 % GemmStage<T>::VL == 0) && (lda % GemmStage<T>::VL == 0);
+    st.load(A, lda, n, rbase, cbase, k0, min(kKC, k1 - k0), vec);
+    for (int kc = k0; kc < k1; kc += kKC) {
+        __syncthreads();  // the previous chunk's reads of As/Bs are done
+        st.store(As, Bs);
         __syncthreads();
+        if (kc + kKC < k1) st.load(A, lda, n, rbase, cbase, kc + kKC, min(kKC, k1 - kc - kKC), vec);
+#pragma unroll
+        for (int ks = 0; ks < kKC; ks += 4) {
+            T a[4], b[JB];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[ks + lk][wr + 16 * i + lr];
+#pragma unroll
+            for (int j = 0; j < JB; ++j) b[j] = Bs[ks + lk][wc + 16 * j + lr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < JB; ++j) {
+                    if constexpr (kF64)
+                        mfma_16x16x4(b[j], a[i], acc[i][j]);
+                    else
+                        mfma_16x16x4(a[i], b[j], acc[i][j]);
+                }
+        }
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < JB; ++j) {
+            if constexpr (kF64) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = rbase + wr + i * 16 + 4 * (lane >> 4) + r;
-                const int col = cbase + wc + j * 16 + (lane & 15);
-                if (row < n && col < n) {
-                    float* p = A + (int64_t)col * lda + row;
-                    *p -= acc[i][j][r];
+                for (int r = 0; r < 4; ++r) {
+                    const int row = rbase + wr + 16 * i + lr;
+                    const int col = cbase + wc + 16 * j + lk + 4 * r;
+                    if (row < n && col < n) A[(int64_t)col * lda + row] -= acc[i][j][r];
+                }
+            } else {
+                const int row0 = rbase + wr + 16 * i + 4 * lk;
+                const int col = cbase + wc + 16 * j + lr;
+                if (col < n) {
+                    T* p = A + (int64_t)col * lda + row0;
+                    if (row0 + 3 < n && (row0 & 3) == 0) {
+                        f4 c = *reinterpret_cast<f4*>(p);
+                        c -= acc[i][j];
+                        *reinterpret_cast<f4*>(p) = c;
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            if (row0 + r < n) p[r] -= acc[i][j][r];
+                    }
                 }
             }
+        }
 }
 
 // ---- solves with the factors (FP64 vectors) ------------------------------
@@ -368,37 +530,36 @@ __global__ void trsv_update_kernel(const T* A, int64_t lda, double* x, int k0, i
 }
 
 template <typename T>
-hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, hipStream_t s)
+hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, void* ws,
+                       hipStream_t s)
 {
+    double* pval = static_cast<double*>(ws);
+    int* pidx = reinterpret_cast<int*>(pval + kColMaxBlocks);
+    unsigned int* counter = reinterpret_cast<unsigned int*>(pidx + kColMaxBlocks);
     for (int k0 = 0; k0 < n; k0 += nb) {
         const int k1 = std::min(n, k0 + nb);
-        for (int k = k0; k < k1; ++k) {
-            hipLaunchKernelGGL(lu_pivot_kernel<T>, dim3(1), dim3(kPivThreads), 0, s, A, lda, n, k, k0, k1, ipiv, info);
-            if (k + 1 < k1) {
-                const int rows = n - k - 1;
-                if (rows > 0)
-                    hipLaunchKernelGGL(lu_panel_update_kernel<T>, dim3((rows + 255) / 256), dim3(256), 0, s, A, lda,
-                                       n, k, k1);
-            }
+        for (int j = k0; j < k1; ++j) {
+            const int rows = n - j;
+            const int blocks = std::max(1, std::min(kColMaxBlocks, (rows + kColThreads - 1) / kColThreads));
+            hipLaunchKernelGGL(lu_crout_kernel<T>, dim3(blocks), dim3(kColThreads), 0, s, A, lda, n, k0, k1, j, ipiv,
+                               info, pval, pidx, counter);
         }
+        if (k1 < n)
+            hipLaunchKernelGGL(lu_scale_kernel<T>, dim3((n - k1 + 255) / 256), dim3(256), 0, s, A, lda, n, k1 - 1);
         const int outside = n - (k1 - k0);
         if (outside > 0)
             hipLaunchKernelGGL(lu_laswp_kernel<T>, dim3((outside + 255) / 256), dim3(256), 0, s, A, lda, n, k0, k1,
                                ipiv);
         if (k1 < n) {
-            hipLaunchKernelGGL((lu_trsm_kernel<T, 64>), dim3((n - k1 + 255) / 256), dim3(256), 0, s, A, lda, n, k0,
-                               k1);
-            const int tiles = (n - k1 + kTile - 1) / kTile;
-            dim3 grid(tiles, tiles);
+            hipLaunchKernelGGL((lu_trsm_kernel<T, 64>), dim3((n - k1 + kTrsmCols - 1) / kTrsmCols), dim3(kTrsmCols),
+                               0, s, A, lda, n, k0, k1);
             if (use_mfma) {
-                if (sizeof(T) == 8)
-                    hipLaunchKernelGGL(lu_gemm_mfma_f64_kernel, grid, dim3(256), 0, s, reinterpret_cast<double*>(A),
-                                       lda, n, k0, k1);
-                else
-                    hipLaunchKernelGGL(lu_gemm_mfma_f32_kernel, grid, dim3(256), 0, s, reinterpret_cast<float*>(A),
-                                       lda, n, k0, k1);
+                const int rem = n - k1, bn = gemm_bn<T>();
+                hipLaunchKernelGGL(lu_gemm_mfma_kernel<T>, dim3((rem + kBM - 1) / kBM, (rem + bn - 1) / bn), dim3(256),
+                                   0, s, A, lda, n, k0, k1);
             } else {
-                hipLaunchKernelGGL(lu_gemm_valu_kernel<T>, grid, dim3(256), 0, s, A, lda, n, k0, k1);
+                const int tiles = (n - k1 + kTile - 1) / kTile;
+                hipLaunchKernelGGL(lu_gemm_valu_kernel<T>, dim3(tiles, tiles), dim3(256), 0, s, A, lda, n, k0, k1);
             }
         }
         hipError_t e = hipGetLastError();
@@ -467,12 +628,15 @@ hipError_t mfma_probe(int mode, double* out64, float* out32, hipStream_t s)
     return hipGetLastError();
 }
 
+size_t lu_workspace_bytes() { return kColMaxBlocks * (sizeof(double) + sizeof(int)) + 64; }
+
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     hipStream_t s)
+                     void* ws, hipStream_t s)
 {
-    if (nb < 1 || nb > 64) return hipErrorInvalidValue;
-    if (elem_bytes == 8) return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, s);
-    return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, s);
+    if (nb < 1 || nb > 64 || !ws || lda % 4 != 0) return hipErrorInvalidValue;
+    if (elem_bytes == 8)
+        return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, ws, s);
+    return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, ws, s);
 }
 
 hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s)

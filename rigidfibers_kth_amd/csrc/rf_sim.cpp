@@ -438,28 +438,30 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
     }
 }
 
-// Direct solve (dgesv, fortran/coeffs_and_vels.f90:62,158): LU with partial
-// pivoting of a copy of A, then the two triangular solves.  In the mixed mode
-// the FP32 matrix is factorised in FP32 (matrix-core trailing updates) and the
-// solution is refined against the FP64-arithmetic residual b - A x.
-int rf_sim::solve_lu(double* be_out)
+// LU factors of the stored matrix in d_LU / d_ipiv (once per assembled
+// system: *done is set and checked).
+int rf_sim::factorize(bool* done)
 {
+    if (*done) return RF_OK;
     const int es = mixed ? 4 : 8;
     if (!d_LU) {
         void* q = nullptr;
         if (hipMalloc(&q, (size_t)lda * n * es) != hipSuccess) {
-            rf::set_error("solve_lu: cannot allocate the LU workspace");
+            rf::set_error("cannot allocate the LU workspace");
             return RF_ENOMEM;
         }
         allocs.push_back(q);
         d_LU = q;
-        if (hipMalloc(&q, sizeof(int) * (n + 1)) != hipSuccess) {
-            rf::set_error("solve_lu: cannot allocate pivots");
+        const size_t piv_bytes = (sizeof(int) * (n + 1) + 255) / 256 * 256;
+        if (hipMalloc(&q, piv_bytes + lu_workspace_bytes()) != hipSuccess) {
+            rf::set_error("cannot allocate pivots");
             return RF_ENOMEM;
         }
         allocs.push_back(q);
         d_ipiv = static_cast<int*>(q);
         d_info = d_ipiv + n;
+        d_lu_ws = static_cast<char*>(q) + piv_bytes;
+        HIP_TRY(hipMemsetAsync(d_lu_ws, 0, lu_workspace_bytes(), stream));
     }
     HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
     HIP_TRY(hipMemsetAsync(d_info, 0, sizeof(int), stream));
@@ -470,7 +472,7 @@ int rf_sim::solve_lu(double* be_out)
         if ((rc0 = timed_pair(&ea, &eb))) return rc0;
         HIP_TRY(hipEventRecord(ea, stream));
     }
-    HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, stream));
+    HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, d_lu_ws, stream));
     if (p.profile) {
         HIP_TRY(hipEventRecord(eb, stream));
         timed_push(ea, eb, kLU);
@@ -482,9 +484,23 @@ int rf_sim::solve_lu(double* be_out)
     resolve_timed();
     std::memcpy(&info, h_dots, sizeof(int));
     if (info != 0) {
-        rf::set_error("solve_lu: matrix is singular (U(" + std::to_string(info) + "," + std::to_string(info) +
-                      ") = 0)");
+        rf::set_error("LU: matrix is singular (U(" + std::to_string(info) + "," + std::to_string(info) + ") = 0)");
         return RF_ENOCONV;
+    }
+    *done = true;
+    return RF_OK;
+}
+
+// Direct solve (dgesv, fortran/coeffs_and_vels.f90:62,158): LU with partial
+// pivoting of a copy of A, then the two triangular solves.  In the mixed mode
+// the FP32 matrix is factorised in FP32 (matrix-core trailing updates) and the
+// solution is refined against the FP64-arithmetic residual b - A x.
+int rf_sim::solve_lu(double* be_out)
+{
+    const int es = mixed ? 4 : 8;
+    {
+        int rcf = factorize(&lu_factored);
+        if (rcf) return rcf;
     }
     int rc;
     double bn2;
@@ -628,10 +644,30 @@ int rf_sim::learned_guess()
     return RF_OK;
 }
 
-// Neighbour-coupled block-Jacobi (restricted additive Schwarz) right
-// preconditioner; built from the assembled matrix of this step.
+// Right preconditioners of GMRES, built from the assembled matrix of this
+// step: the neighbour-coupled block-Jacobi (restricted additive Schwarz)
+// inverse, or the LU factors of the stored matrix (FP32 factors in the
+// mixed mode: GMRES on A (LU)^-1 with FP64 residuals is then the
+// iterative refinement of the FP32 factorisation).
 int rf_sim::setup_precond()
 {
+    if (!d_z) {
+        void* q = nullptr;
+        for (double** v : {&d_z, &d_t}) {
+            if (hipMalloc(&q, sizeof(double) * (size_t)lda) != hipSuccess) {
+                rf::set_error("cannot allocate preconditioner vectors");
+                return RF_ENOMEM;
+            }
+            allocs.push_back(q);
+            *v = static_cast<double*>(q);
+        }
+    }
+    int rc;
+    if (p.precond == RF_PRECOND_LU) {
+        if ((rc = factorize(&lu_factored))) return rc;
+        prec_ready = true;
+        return RF_OK;
+    }
     const int bs = 3 * M;
     const int dim = bs * (ras_k + 1);
     if (!d_B) {
@@ -642,14 +678,6 @@ int rf_sim::setup_precond()
         }
         allocs.push_back(q);
         d_B = static_cast<double*>(q);
-        for (double** v : {&d_z, &d_t}) {
-            if (hipMalloc(&q, sizeof(double) * (size_t)lda) != hipSuccess) {
-                rf::set_error("cannot allocate preconditioner vectors");
-                return RF_ENOMEM;
-            }
-            allocs.push_back(q);
-            *v = static_cast<double*>(q);
-        }
         if (hipMalloc(&q, sizeof(int)) != hipSuccess) {
             rf::set_error("cannot allocate preconditioner flag");
             return RF_ENOMEM;
@@ -657,8 +685,7 @@ int rf_sim::setup_precond()
         allocs.push_back(q);
         d_pinfo = static_cast<int*>(q);
     }
-    int rc = setup_neighbours();
-    if (rc) return rc;
+    if ((rc = setup_neighbours())) return rc;
     HIP_TRY(hipMemsetAsync(d_pinfo, 0, sizeof(int), stream));
     HIP_TRY(launch_ras_setup(N, ras_k, knn, bs, d_nbr, d_A, mixed ? 4 : 8, lda, d_B, d_pinfo, stream));
     int info = 0;
@@ -676,6 +703,11 @@ int rf_sim::setup_precond()
 
 int rf_sim::precond_apply(const double* r, double* z)
 {
+    if (p.precond == RF_PRECOND_LU) {
+        if (r != z) HIP_TRY(hipMemcpyAsync(z, r, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(lu_solve(d_LU, mixed ? 4 : 8, lda, n, d_ipiv, z, stream));
+        return RF_OK;
+    }
     HIP_TRY(launch_ras_apply(N, ras_k, knn, 3 * M, d_nbr, d_B, r, z, stream));
     return RF_OK;
 }
@@ -728,6 +760,7 @@ int rf_sim::assemble()
     stats.ms_assemble = phase_end();
     stats.total_ms_assemble += stats.ms_assemble;
     assembled = true;
+    lu_factored = false;
     return RF_OK;
 }
 
@@ -759,7 +792,7 @@ int rf_sim::solve()
         stats.last_solver = RF_SOLVER_LU;
     } else {
         prec_ready = false;
-        if (p.precond == RF_PRECOND_NEIGHBOUR && (rc = setup_precond())) return rc;
+        if (p.precond != RF_PRECOND_NONE && (rc = setup_precond())) return rc;
         rc = solve_gmres(&iters, &be, &status);
         if (rc) return rc;
         stats.last_solver = RF_SOLVER_GMRES;
@@ -858,10 +891,10 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     if (num_fibers < 1 || P.num_terms < 1 || P.num_terms > kMaxTerms || P.num_quad_intervals < 1 ||
         3 * P.num_quad_intervals > kMaxQuad || !(P.slenderness > 0.0) || P.gmres_restart < 1 ||
         P.gmres_restart > 60 || P.gmres_max_iters < 1 || (P.precision != RF_PREC_FP64 && P.precision != RF_PREC_MIXED) ||
-        P.precond < 0 || P.precond > 1 || P.precond_neighbours < 0 || P.precond_neighbours > 6 ||
-        (P.precond && num_fibers < 2)) {
+        P.precond < 0 || P.precond > 2 || P.precond_neighbours < 0 || P.precond_neighbours > 6 ||
+        (P.precond == RF_PRECOND_NEIGHBOUR && num_fibers < 2)) {
         set_error("rf_create: invalid parameters (need N>=1, 1<=M<=8, 1<=NoQI<=32, eps>0, 1<=restart<=60, "
-                  "precond 0|1 with 0<=precond_neighbours<=6 and N>=2)");
+                  "precond 0|1|2, 0<=precond_neighbours<=6, N>=2 for the neighbour preconditioner)");
         return RF_EINVAL;
     }
     int dev = 0, ndev = 0;
@@ -877,7 +910,7 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     s->n = 3 * s->M * s->N;
     s->mixed = (P.precision == RF_PREC_MIXED) ? 1 : 0;
     s->restart = P.gmres_restart;
-    s->ras_k = P.precond ? std::min(num_fibers - 1, P.precond_neighbours > 0 ? P.precond_neighbours : 4) : 0;
+    s->ras_k = (P.precond == RF_PRECOND_NEIGHBOUR) ? std::min(num_fibers - 1, P.precond_neighbours > 0 ? P.precond_neighbours : 4) : 0;
     // neighbour lists keep the 16 nearest fibers (the network's input); the
     // preconditioner uses the first ras_k of them
     s->knn = std::min(num_fibers - 1, kMlpNeighbours);

@@ -82,6 +82,8 @@ struct rf_sim {
     void* d_LU = nullptr;
     int* d_ipiv = nullptr;
     int* d_info = nullptr;
+    void* d_lu_ws = nullptr;   // pivot-search partials and counter
+    bool lu_factored = false;  // d_LU holds the factors of the current matrix
 
     rf_stats stats{};
 
@@ -110,6 +112,7 @@ struct rf_sim {
     int norm2(const double* v, double* out);
     int solve_gmres(int* iters, double* be, int* status);
     int solve_lu(double* be);
+    int factorize(bool* done);
     int setup_neighbours();
     int mlp_alloc();
     int mlp_infer(double* x_out);

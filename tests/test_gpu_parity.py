@@ -139,11 +139,13 @@ def test_mfma_16x16x4_lane_layout():
     assert all(report.values()), (report, got[0][1][:20:4], got[1][1][:20:4])
 
 
-@pytest.mark.parametrize("valu", [1, 0])
-def test_lu_solution_vs_oracle(valu):
+@pytest.mark.parametrize("valu,nb", [(1, 0), (0, 0), (0, 40), (0, 13)])
+def test_lu_solution_vs_oracle(valu, nb):
+    """Blocked LU (panel width nb, 0 = 64; partial last panels and trailing
+    tiles) on the vector ALUs or the matrix cores vs the oracle's dgesv."""
     N = 30
     pos, ori = _layout(N, 2e-2, 13)
-    with rf.Simulation(rf.Configuration(_params(1, 1, lu_valu_update=valu), pos, ori)) as sim:
+    with rf.Simulation(rf.Configuration(_params(1, 1, lu_valu_update=valu, lu_block=nb), pos, ori)) as sim:
         sim.assemble_system()
         sim.solve_system()
         x = sim.coefficients()

@@ -125,3 +125,25 @@ def test_mixed_lu_with_refinement():
     assert st.last_backward_error <= 1e-12
     assert np.linalg.norm(b - A32 @ x32) <= 1e-11 * np.linalg.norm(b)
     assert np.linalg.norm(x32 - x64) <= 1e-5 * np.linalg.norm(x64)
+
+
+@pytest.mark.parametrize("precision", [rf.PREC_FP64, rf.PREC_MIXED])
+def test_lu_preconditioned_gmres(precision):
+    """GMRES right-preconditioned by the LU factors of the stored matrix
+    (FP32 factors in the mixed mode, i.e. GMRES-based iterative refinement
+    with FP64 residuals) reaches the reference's GMRES solution of the same
+    stored operator in a few iterations, also on the near-contact state."""
+    pos, ori = _hard_state()
+    out = {}
+    for pc in (rf.PRECOND_NONE, rf.PRECOND_LU):
+        with _sim(pos, ori, precond=pc, precision=precision) as sim:
+            sim.assemble_system()
+            sim.solve_system()
+            st = sim.stats()
+            out[pc] = (sim.coefficients(), st.last_gmres_iters, st.last_backward_error, st.last_solver)
+    x0, it0, _, _ = out[rf.PRECOND_NONE]
+    x1, it1, be1, s1 = out[rf.PRECOND_LU]
+    assert s1 == rf.SOLVER_GMRES and be1 <= 1e-10
+    assert it1 <= (3 if precision == rf.PREC_FP64 else 20), it1
+    assert it1 * 10 <= it0, (it0, it1)
+    assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)

@@ -1,19 +1,31 @@
 #!/usr/bin/env python3
 """Throughput of the rigid-fiber slender-body timestep on one MI355X.
 
-Workload (the plain FP64 GPU configuration): N=1000 rigid fibers of length 2,
-slenderness 1e-2, centres uniform in a cube sized for volume fraction 1e-2,
-orientations uniform on the sphere (seeded), gravity (0,0,-1); M=5 Legendre
-terms, 8x3 Gauss points, analytic inner integrals, FP64, GMRES(10) to a
-backward error of 1e-10 warm-started from the previous step, dt=0.1.
+Headline workload (`value`): N=1000 rigid fibers of length 2, slenderness
+1e-2, centres uniform in a cube sized for volume fraction 1e-2, orientations
+uniform on the sphere (seeded, overlaps rejected), gravity (0,0,-1); M=5
+Legendre terms, 8x3 Gauss points, analytic inner integrals, dt=0.1; solved the
+reference's way in FP64: GMRES(10) to a backward error of 1e-10, warm-started
+from the previous step, LU fallback (fortran/coeffs_and_vels.f90:253-339).
 A "step" is one full timestep (assemble, solve, velocities, advance).
 
-Prints one JSON line.  `value` is steps/s of K timed steps on one GPU with all
-state resident in HBM.  `roofline` describes the dominant kernel (the
-system-matrix GEMV of the GMRES iterations), timed with HIP events on the
-library's stream inside the timed region.  `cpu_baseline` times the FP64 CPU
-oracle (a C restatement of the reference's Fortran) on the host cores on one
-step of the same workload.
+Beside it, on the same initial state: the mixed-precision + learned path
+(FP32 matrix with FP64 near field and closed forms, FP64 residuals and
+Arnoldi, neighbour preconditioner switching to FP32-LU GMRES-IR on hard
+systems, learned initial guess trained on the GPU on solved pairs), its
+relative position error against the plain run after the same steps, the FP64
+direct solver, and the same comparison at N=4000, volume fraction 5e-2.
+
+The position errors are reported next to the difference between two FP64
+runs whose solves stop at 1e-10 and 1e-12: near fiber contacts the dynamics
+amplify any difference in a solve, so that pair bounds what agreement any
+solver can reach after the same number of steps.
+
+Prints one JSON line.  `roofline` describes the dominant kernel of the
+headline workload (the system-matrix GEMV of the GMRES iterations), timed with
+HIP events on the library's stream inside the timed region.  `cpu_baseline`
+times the FP64 CPU oracle (a C restatement of the reference's Fortran) on the
+host cores on one step of the headline workload.
 """
 import argparse
 import json
@@ -29,7 +41,20 @@ import numpy as np  # noqa: E402
 METRIC = "steps/sec at N=1000/4000 fibers: plain FP64 vs mixed+learned at ≤1e-6 rel err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemv_f64_n15000_traffic.json")
+
+C2 = dict(N=1000, phi=1e-2, seed=0x5EED0001)
+C4 = dict(N=4000, phi=5e-2, seed=0x5EED0003)
+# the reference's method: FP64 GMRES(10), previous-step guess, LU fallback
+PLAIN = dict(gmres_restart=10, gmres_max_iters=3000, gmres_tol=1e-10)
+# mixed precision + learned: FP64 operator, residuals and Arnoldi; FP32 LU
+# factors on the matrix cores as the preconditioner of hard systems (GMRES-IR)
+# after two neighbour-preconditioned restart cycles; learned initial guess
+IMPROVED = dict(gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-10, precond=3, lu_factor_fp32=1,
+                warm_start=3)
+# the same without the network (previous-step guess)
+IMPROVED_NO_MLP = dict(IMPROVED, warm_start=0)
 
 
 def segment_distances(c, t, C, T, half=1.0):
@@ -103,27 +128,9 @@ def analytic_assembly_flops(pos, ori, M=5, noqi=8):
     return float(flops), near / total_pts
 
 
-def run_gpu(N, phi, seed, steps, warmup, **kw):
-    import rigidfibers_kth_amd as rf
-
-    pos, ori = layout(N, phi, seed)
-    p = rf.default_params(gmres_restart=10, gmres_max_iters=2000, gmres_tol=1e-10, profile=1, **kw)
-    sim = rf.Simulation(rf.Configuration(p, pos, ori))
-    for _ in range(warmup):
-        sim.step()
-    sim.synchronize()
-    s0 = sim.stats()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.step()
-    sim.synchronize()
-    t1 = time.perf_counter()
-    s1 = sim.stats()
-    wall = t1 - t0
+def _stats_delta(s0, s1, steps):
     d = lambda f: getattr(s1, f) - getattr(s0, f)  # noqa: E731
-    out = {
-        "steps_per_sec": steps / wall,
-        "ms_per_step": 1e3 * wall / steps,
+    return {
         "phases_ms_per_step": {
             "assemble": d("total_ms_assemble") / steps,
             "solve": d("total_ms_solve") / steps,
@@ -131,19 +138,103 @@ def run_gpu(N, phi, seed, steps, warmup, **kw):
             "advance": d("total_ms_advance") / steps,
         },
         "gmres_iters_per_step": d("total_gmres_iters") / steps,
-        "last_backward_error": s1.last_backward_error,
         "gemv_launches": d("gemv_launches"),
         "gemv_avg_ms": d("gemv_ms_total") / max(1, d("gemv_launches")),
         "gemv_total_ms": d("gemv_ms_total"),
         "assemble_avg_ms": d("assemble_ms_total") / max(1, d("assemble_launches")),
         "velocity_gemv_avg_ms": d("velocity_gemv_ms_total") / max(1, d("velocity_gemv_launches")),
+        "lu_factorizations": d("lu_factorizations"),
         "lu_avg_ms": d("lu_ms_total") / max(1, d("lu_factorizations")),
-        "n": sim.n,
-        "lda": s1.lda,
-        "elem_bytes": s1.matrix_elem_bytes,
     }
+
+
+def timed_run(rf, pos, ori, steps, warmup=0, weights=None, **kw):
+    """K timed steps after W untimed ones; returns (record, final positions)."""
+    sim = rf.Simulation(rf.Configuration(rf.default_params(profile=1, **kw), pos, ori))
+    if weights is not None:
+        sim.mlp_set_weights(weights)
+    for _ in range(warmup):
+        sim.step()
+    sim.synchronize()
+    s0 = sim.stats()
+    t0 = time.perf_counter()
+    first = None
+    for k in range(steps):
+        sim.step()
+        if k == 0:  # positions after the first timed step (one 3N copy)
+            first, _ = sim.state()
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    s1 = sim.stats()
+    out = {"steps_per_sec": steps / wall, "ms_per_step": 1e3 * wall / steps, "steps": steps,
+           "last_backward_error": s1.last_backward_error, "n": sim.n, "elem_bytes": s1.matrix_elem_bytes}
+    out.update(_stats_delta(s0, s1, steps))
+    P, _ = sim.state()
     sim.close()
-    return out, pos, ori
+    return out, (P, first)
+
+
+def train_guess(rf, pos, ori, steps, epochs=30, **kw):
+    """Solved (configuration, coefficients) pairs along `steps` steps from
+    (pos, ori), then GPU training of the network; returns (weights, info)."""
+    sim = rf.Simulation(rf.Configuration(rf.default_params(**kw), pos, ori))
+    Xs, Ys = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.assemble_system()
+        sim.solve_system()
+        X, Y = sim.mlp_samples()
+        Xs.append(X)
+        Ys.append(Y)
+        sim.update_velocities()
+        sim.update_fibers()
+    t1 = time.perf_counter()
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    hist = sim.mlp_train(X, Y, epochs=epochs, batch=256, lr=1e-3, seed=1)
+    sim.synchronize()
+    t2 = time.perf_counter()
+    w = sim.mlp_weights()
+    P, T = sim.state()
+    sim.close()
+    return w, {"samples": int(X.shape[0]), "epochs": epochs, "sample_s": t1 - t0, "train_s": t2 - t1,
+               "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1])}, P, T
+
+
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def compare(rf, cfg, steps, warmup, train_states, label):
+    """Plain FP64 and mixed + learned on the same state: steps/s, ratio and
+    position error after the same number of steps, next to the error of a run
+    at a 100x tighter tolerance (the trajectories are chaotic near fiber
+    contacts: any 1e-10 difference in a solve grows step by step)."""
+    pos, ori = layout(cfg["N"], cfg["phi"], cfg["seed"])
+    # the network is trained on solved pairs of the first states of the
+    # trajectory; the timed runs start where the training states end
+    w, tinfo, P0, T0 = train_guess(rf, pos, ori, train_states, **IMPROVED_NO_MLP)
+    tag = label.split(",")[0]
+    log("%s: network trained on %d pairs" % (tag, tinfo["samples"]))
+    plain, Xp = timed_run(rf, P0, T0, steps, warmup, **PLAIN)
+    log("%s: plain FP64 %.3f steps/s" % (tag, plain["steps_per_sec"]))
+    impr, Xi = timed_run(rf, P0, T0, steps, warmup, weights=w, **IMPROVED)
+    log("%s: mixed + learned %.3f steps/s" % (tag, impr["steps_per_sec"]))
+    tight, Xt = timed_run(rf, P0, T0, steps, warmup, **dict(IMPROVED_NO_MLP, gmres_tol=1e-12))
+    return {
+        "workload": label,
+        "plain_fp64": plain,
+        "mixed_learned": impr,
+        "speedup": impr["steps_per_sec"] / plain["steps_per_sec"],
+        "position_rel_err": rel(Xi[0], Xp[0]),
+        "position_rel_err_floor": rel(Xt[0], Xp[0]),
+        "position_rel_err_one_step": rel(Xi[1], Xp[1]),
+        "training": tinfo,
+    }
 
 
 def cpu_baseline(N, phi, seed, threads):
@@ -159,7 +250,7 @@ def cpu_baseline(N, phi, seed, threads):
         "unit": "steps/s",
         "cores": int(oracle.num_threads()),
         "kind": "port",
-        "sample": "1 timestep of the same N=1000 workload (analytic assembly, GMRES(10) to 1e-10 from x0=0, "
+        "sample": "1 timestep of the N=1000 headline workload (analytic assembly, GMRES(10) to 1e-10 from x0=0, "
                   "velocities, advance) with the FP64 C oracle, OpenMP over target fibers; %.1f s" % dt,
     }
 
@@ -167,18 +258,23 @@ def cpu_baseline(N, phi, seed, threads):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-lu", action="store_true", help="skip the direct-solver timing")
+    ap.add_argument("--no-n4000", action="store_true", help="skip the N=4000 comparison")
+    ap.add_argument("--n4000-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the multi-GPU row sharding is a design only "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
-    N, phi, seed = 1000, 1e-2, 0x5EED0001
-    g, pos, ori = run_gpu(N, phi, seed, args.steps, args.warmup)
-    n, lda, eb = g["n"], g["lda"], g["elem_bytes"]
+    import rigidfibers_kth_amd as rf
+
+    pos, ori = layout(C2["N"], C2["phi"], C2["seed"])
+    g, Xp = timed_run(rf, pos, ori, args.steps, args.warmup, **PLAIN)
+    log("N=1000 plain FP64: %.2f steps/s" % g["steps_per_sec"])
+    n, eb = g["n"], g["elem_bytes"]
     gemv_bytes = float(n) * n * eb + 2.0 * n * 8  # matrix once + x and y
     achieved = gemv_bytes / (g["gemv_avg_ms"] * 1e-3) / 1e9
     traffic = None
@@ -187,16 +283,45 @@ def main():
             t = json.load(open(TRAFFIC_FILE))
             if t.get("n") == n and t.get("elem_bytes") == eb:
                 traffic = t.get("bytes_per_launch")
-        except Exception:
+        except (OSError, ValueError):
             traffic = None
     flops, near_frac = analytic_assembly_flops(pos, ori)
     asm_tflops = flops / (g["assemble_avg_ms"] * 1e-3) / 1e12
-    extra = {}
+
+    # mixed + learned at N=1000 on the same initial state and step count; the
+    # network is trained on solved pairs of another seeded layout
+    w, tinfo, _, _ = train_guess(rf, *layout(C2["N"], C2["phi"], C2["seed"] + 100), 6, **IMPROVED_NO_MLP)
+    impr, Xi = timed_run(rf, pos, ori, args.steps, args.warmup, weights=w, **IMPROVED)
+    log("N=1000 mixed + learned: %.2f steps/s" % impr["steps_per_sec"])
+    nomlp, _ = timed_run(rf, pos, ori, args.steps, args.warmup, **IMPROVED_NO_MLP)
+    tight, Xt = timed_run(rf, pos, ori, args.steps, args.warmup, **dict(PLAIN, gmres_tol=1e-12))
+    log("N=1000 floor run: %.2f steps/s" % tight["steps_per_sec"])
+    extra = {
+        "n1000_mixed_learned": dict(impr, position_rel_err=rel(Xi[0], Xp[0]),
+                                    position_rel_err_floor=rel(Xt[0], Xp[0]),
+                                    position_rel_err_one_step=rel(Xi[1], Xp[1]),
+                                    speedup=impr["steps_per_sec"] / g["steps_per_sec"], training=tinfo),
+        "n1000_without_network": {"steps_per_sec": nomlp["steps_per_sec"],
+                                  "gmres_iters_per_step": nomlp["gmres_iters_per_step"]},
+    }
     if not args.no_lu:
-        lu, _, _ = run_gpu(N, phi, seed, 2, 1, use_direct_solver=1)
-        extra["lu_steps_per_sec"] = lu["steps_per_sec"]
-        extra["lu_factor_ms"] = lu["lu_avg_ms"]
-        extra["lu_factor_tflops"] = (2.0 / 3.0) * n ** 3 / (lu["lu_avg_ms"] * 1e-3) / 1e12
+        lu, _ = timed_run(rf, pos, ori, 2, 1, use_direct_solver=1)
+        log("N=1000 FP64 LU: %.1f ms per factorisation" % lu["lu_avg_ms"])
+        extra["n1000_lu_fp64"] = {
+            "steps_per_sec": lu["steps_per_sec"],
+            "lu_factor_ms": lu["lu_avg_ms"],
+            "lu_tflops": (2.0 / 3.0) * n ** 3 / (lu["lu_avg_ms"] * 1e-3) / 1e12,
+            "frac_fp64_peak": (2.0 / 3.0) * n ** 3 / (lu["lu_avg_ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        }
+    if not args.no_n4000:
+        c4 = compare(rf, C4, args.n4000_steps, 0, 3, "N=4000, volume fraction 5e-2, M=5, dt=0.1, tol 1e-10; "
+                     "timed steps start after 3 solved training states")
+        mi = c4["mixed_learned"]
+        if mi["lu_factorizations"]:
+            n4 = mi["n"]
+            tf = (2.0 / 3.0) * n4 ** 3 / (mi["lu_avg_ms"] * 1e-3) / 1e12
+            c4["fp32_lu"] = {"factor_ms": mi["lu_avg_ms"], "tflops": tf, "frac_fp32_mfma_peak": tf / FP32_MFMA_PEAK_TFLOPS}
+        extra["n4000"] = c4
     line = {
         "metric": METRIC,
         "value": g["steps_per_sec"],
@@ -209,11 +334,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: seeded fiber layout (seed 0x5EED0001)",
+        "data": "synthetic: seeded fiber layouts (N=1000 seed 0x5EED0001, N=4000 seed 0x5EED0003), overlaps rejected",
         "config": {
             "workload": "plain FP64, N=1000 fibers, volume fraction 1e-2, M=5, 8x3 Gauss points, analytic, "
-                        "GMRES(10) tol 1e-10, dt=0.1",
-            "fibers": N, "unknowns": n, "volume_fraction": phi, "solver": "gmres", "tol": 1e-10,
+                        "GMRES(10) tol 1e-10 with LU fallback, previous-step warm start, dt=0.1",
+            "fibers": C2["N"], "unknowns": n, "volume_fraction": C2["phi"], "solver": "gmres(10)", "tol": 1e-10,
         },
         "roofline": {
             "kernel": "gemv_parts_kernel<double> (GMRES matvec, n=%d)" % n,
@@ -238,7 +363,8 @@ def main():
     }
     line.update(extra)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(N, phi, seed, min(16, os.cpu_count() or 1))
+        log("CPU oracle step")
+        line["cpu_baseline"] = cpu_baseline(C2["N"], C2["phi"], C2["seed"], min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)
 
 

@@ -54,6 +54,7 @@ extern "C" {
 #define RF_PRECOND_NONE 0
 #define RF_PRECOND_NEIGHBOUR 1 /* block-Jacobi on each fiber + its K nearest fibers (restricted additive Schwarz) */
 #define RF_PRECOND_LU 2        /* LU factors of the stored matrix (FP32 in the mixed mode: GMRES-IR) */
+#define RF_PRECOND_AUTO 3      /* neighbour preconditioner; after precond_switch_iters iterations the LU */
 
 /* output number formats */
 #define RF_FMT_F16_8 0  /* '(*(F16.8))' as the Fortran's *.out / tests/reference *.ref */
@@ -92,7 +93,9 @@ typedef struct rf_params {
     int32_t lu_valu_update; /* 1: LU trailing update on the vector ALUs instead of the matrix cores */
     int32_t precond;        /* RF_PRECOND_*: right preconditioner of GMRES */
     int32_t precond_neighbours; /* K nearest fibers per preconditioner cluster (1..6, 0 = 4) */
-    int32_t reserved[5];
+    int32_t precond_switch_iters; /* RF_PRECOND_AUTO: neighbour-preconditioned iterations before the LU (0 = 2 restarts) */
+    int32_t lu_factor_fp32;       /* 1: FP32 LU factors also for the FP64 matrix (preconditioner / refinement) */
+    int32_t reserved[3];
 } rf_params;
 
 typedef struct rf_stats {
@@ -119,7 +122,9 @@ typedef struct rf_stats {
     double last_initial_residual;   /* ||b - A x0|| / ||b|| of the last GMRES solve */
     double last_guess_alpha;        /* learned guess: x0 = alpha x_prev + beta MLP */
     double last_guess_beta;
-    double reserved[5];
+    int32_t last_precond;           /* RF_PRECOND_* in use at the end of the last GMRES solve */
+    int32_t pad0;
+    double reserved[4];
 } rf_stats;
 
 typedef struct rf_sim rf_sim;
@@ -131,7 +136,8 @@ void rf_params_default(rf_params *p);
 /* v1 parameter file (src/parameters.cc:124-180; fortran/advect_fibers.f90:182-222).
  * Lines after the last mandatory one may hold "key value" extensions:
  * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1,
- * lu_block NB, precond none|neighbour|lu, precond_neighbours K, mixed_inner_tol T. */
+ * lu_block NB, precond none|neighbour|lu|auto, precond_neighbours K, precond_switch_iters I,
+ * lu_factor_fp32 0|1, mixed_inner_tol T. */
 int rf_parse_parameters(const char *path, rf_params *out);
 
 /* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per

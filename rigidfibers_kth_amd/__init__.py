@@ -26,7 +26,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 WARM_PREV, WARM_ZERO, WARM_EXTRAP, WARM_LEARNED = 0, 1, 2, 3
 FMT_F16_8, FMT_F24_16 = 0, 1
 SOLVER_GMRES, SOLVER_LU, SOLVER_LU_FALLBACK = 0, 1, 2
-PRECOND_NONE, PRECOND_NEIGHBOUR, PRECOND_LU = 0, 1, 2
+PRECOND_NONE, PRECOND_NEIGHBOUR, PRECOND_LU, PRECOND_AUTO = 0, 1, 2, 3
 
 
 class rf_params(ctypes.Structure):
@@ -55,7 +55,9 @@ class rf_params(ctypes.Structure):
         ("lu_valu_update", ctypes.c_int32),
         ("precond", ctypes.c_int32),
         ("precond_neighbours", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("precond_switch_iters", ctypes.c_int32),
+        ("lu_factor_fp32", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -88,7 +90,9 @@ class rf_stats(ctypes.Structure):
         ("last_initial_residual", ctypes.c_double),
         ("last_guess_alpha", ctypes.c_double),
         ("last_guess_beta", ctypes.c_double),
-        ("reserved", ctypes.c_double * 5),
+        ("last_precond", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("reserved", ctypes.c_double * 4),
     ]
 
     def as_dict(self):
@@ -211,7 +215,10 @@ def selftest_copy(nbytes, reps=10):
 def default_params(**kw):
     p = rf_params()
     lib().rf_params_default(ctypes.byref(p))
+    names = {f[0] for f in rf_params._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError("unknown run parameter %r" % k)
         if k == "external_force":
             for i in range(3):
                 p.external_force[i] = v[i]

@@ -70,6 +70,7 @@ hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double
                        const double* b, int mode, hipStream_t s, hipEvent_t ev0 = nullptr,
                        hipEvent_t ev1 = nullptr);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
 
 // Deterministic reductions / vector kernels (rf_linalg.hip).
 hipError_t launch_reduce_splits(const double* parts, int splits, int64_t stride, int n, double* out,

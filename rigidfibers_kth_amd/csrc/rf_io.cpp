@@ -162,9 +162,14 @@ extern "C" int rf_parse_parameters(const char* path, rf_params* out)
             if (val == "none") p.precond = RF_PRECOND_NONE;
             else if (val == "neighbour") p.precond = RF_PRECOND_NEIGHBOUR;
             else if (val == "lu") p.precond = RF_PRECOND_LU;
+            else if (val == "auto") p.precond = RF_PRECOND_AUTO;
             else ok = false;
         } else if (key == "precond_neighbours") {
             ok = parse_int(val, &p.precond_neighbours);
+        } else if (key == "precond_switch_iters") {
+            ok = parse_int(val, &p.precond_switch_iters);
+        } else if (key == "lu_factor_fp32") {
+            ok = parse_int(val, &p.lu_factor_fp32);
         } else if (key == "mixed_inner_tol") {
             ok = parse_double(val, &p.mixed_inner_tol);
         } else {

@@ -207,7 +207,20 @@ __global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict
         dst[i] = src[i];
 }
 
+__global__ void f64_to_f32_kernel(const double* __restrict__ src, float* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
 }  // namespace
+
+hipError_t launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s)
+{
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s)
 {

@@ -332,10 +332,9 @@ int rf_sim::update_iterate(int k)
 // the same iterates as the reference's modified Gram-Schmidt.  With
 // rf_params.precond the same iteration runs on A M^-1 (right preconditioning,
 // the stopping test still uses the true residual b - A x).
-int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
+int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int maxit)
 {
     const int m = restart;
-    const int maxit = p.gmres_max_iters;
     const double tol = (mixed && p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : p.gmres_tol;
     const int ldh = m + 1;
     std::vector<double> H((size_t)(m + 1) * (m + 2), 0.0), g(m + 1, 0.0), cs(m + 1), sn(m + 1), y(m + 1);
@@ -350,8 +349,11 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
         *be_out = 0.0;
         return RF_OK;
     }
-    // r0 = b - A x
-    if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
+    // r0 = b - A x (the learned guess leaves it in d_r)
+    if (r0_ready)
+        r0_ready = false;
+    else if ((rc = gemv_A(d_x, d_r, d_b, 1)))
+        return rc;
     double beta2;
     if ((rc = norm2(d_r, &beta2))) return rc;
     double beta = std::sqrt(beta2);
@@ -443,7 +445,7 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out)
 int rf_sim::factorize(bool* done)
 {
     if (*done) return RF_OK;
-    const int es = mixed ? 4 : 8;
+    const int es = lu_es;
     if (!d_LU) {
         void* q = nullptr;
         if (hipMalloc(&q, (size_t)lda * n * es) != hipSuccess) {
@@ -463,7 +465,10 @@ int rf_sim::factorize(bool* done)
         d_lu_ws = static_cast<char*>(q) + piv_bytes;
         HIP_TRY(hipMemsetAsync(d_lu_ws, 0, lu_workspace_bytes(), stream));
     }
-    HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
+    if (es == (mixed ? 4 : 8))
+        HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
+    else  // FP32 factors of the FP64 matrix
+        HIP_TRY(launch_f64_to_f32(static_cast<const double*>(d_A), static_cast<float*>(d_LU), (size_t)lda * n, stream));
     HIP_TRY(hipMemsetAsync(d_info, 0, sizeof(int), stream));
     const int nb = (p.lu_block > 0) ? std::min(64, p.lu_block) : 64;
     hipEvent_t ea = nullptr, eb = nullptr;
@@ -497,7 +502,7 @@ int rf_sim::factorize(bool* done)
 // solution is refined against the FP64-arithmetic residual b - A x.
 int rf_sim::solve_lu(double* be_out)
 {
-    const int es = mixed ? 4 : 8;
+    const int es = lu_es;
     {
         int rcf = factorize(&lu_factored);
         if (rcf) return rcf;
@@ -506,7 +511,7 @@ int rf_sim::solve_lu(double* be_out)
     double bn2;
     if ((rc = norm2(d_b, &bn2))) return rc;
     const double bn = std::sqrt(bn2);
-    if (!mixed) {
+    if (es == 8) {
         HIP_TRY(hipMemcpyAsync(d_x, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
         HIP_TRY(lu_solve(d_LU, es, lda, n, d_ipiv, d_x, stream));
         if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
@@ -641,6 +646,10 @@ int rf_sim::learned_guess()
     stats.last_guess_alpha = alpha;
     stats.last_guess_beta = beta;
     HIP_TRY(launch_axpby(n, alpha, d_xprev, beta, d_guess, d_x, stream));
+    // r0 = b - alpha A x_prev - beta A g, by linearity, for the GMRES start
+    HIP_TRY(launch_axpby(n, 1.0, bb, -alpha, u, d_r, stream));
+    HIP_TRY(launch_axpby(n, 1.0, d_r, -beta, v, d_r, stream));
+    r0_ready = true;
     return RF_OK;
 }
 
@@ -649,7 +658,7 @@ int rf_sim::learned_guess()
 // inverse, or the LU factors of the stored matrix (FP32 factors in the
 // mixed mode: GMRES on A (LU)^-1 with FP64 residuals is then the
 // iterative refinement of the FP32 factorisation).
-int rf_sim::setup_precond()
+int rf_sim::setup_precond(int kind)
 {
     if (!d_z) {
         void* q = nullptr;
@@ -663,8 +672,9 @@ int rf_sim::setup_precond()
         }
     }
     int rc;
-    if (p.precond == RF_PRECOND_LU) {
+    if (kind == RF_PRECOND_LU) {
         if ((rc = factorize(&lu_factored))) return rc;
+        prec_kind = RF_PRECOND_LU;
         prec_ready = true;
         return RF_OK;
     }
@@ -697,15 +707,16 @@ int rf_sim::setup_precond()
         rf::set_error("preconditioner: singular cluster matrix at fiber " + std::to_string(info - 1));
         return RF_ENOCONV;
     }
+    prec_kind = RF_PRECOND_NEIGHBOUR;
     prec_ready = true;
     return RF_OK;
 }
 
 int rf_sim::precond_apply(const double* r, double* z)
 {
-    if (p.precond == RF_PRECOND_LU) {
+    if (prec_kind == RF_PRECOND_LU) {
         if (r != z) HIP_TRY(hipMemcpyAsync(z, r, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-        HIP_TRY(lu_solve(d_LU, mixed ? 4 : 8, lda, n, d_ipiv, z, stream));
+        HIP_TRY(lu_solve(d_LU, lu_es, lda, n, d_ipiv, z, stream));
         return RF_OK;
     }
     HIP_TRY(launch_ras_apply(N, ras_k, knn, 3 * M, d_nbr, d_B, r, z, stream));
@@ -787,14 +798,33 @@ int rf_sim::solve()
     int iters = 0, status = 0;
     double be = 0.0;
     if (p.use_direct_solver) {
+        r0_ready = false;
         rc = solve_lu(&be);
         if (rc) return rc;
         stats.last_solver = RF_SOLVER_LU;
     } else {
         prec_ready = false;
-        if (p.precond != RF_PRECOND_NONE && (rc = setup_precond())) return rc;
-        rc = solve_gmres(&iters, &be, &status);
-        if (rc) return rc;
+        prec_kind = RF_PRECOND_NONE;
+        const int maxit = p.gmres_max_iters;
+        if (p.precond == RF_PRECOND_AUTO) {
+            // neighbour preconditioner within a budget, then GMRES on A (LU)^-1
+            // from the iterate reached
+            const int budget = std::min(maxit, p.precond_switch_iters > 0 ? p.precond_switch_iters : 2 * restart);
+            if ((rc = setup_precond(RF_PRECOND_NEIGHBOUR))) return rc;
+            if ((rc = solve_gmres(&iters, &be, &status, budget))) return rc;
+            if (status != 0 && iters < maxit) {
+                const double r0 = stats.last_initial_residual;
+                if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
+                int it2 = 0;
+                if ((rc = solve_gmres(&it2, &be, &status, maxit - iters))) return rc;
+                iters += it2;
+                stats.last_initial_residual = r0;
+            }
+        } else {
+            if (p.precond != RF_PRECOND_NONE && (rc = setup_precond(p.precond))) return rc;
+            if ((rc = solve_gmres(&iters, &be, &status, maxit))) return rc;
+        }
+        stats.last_precond = prec_kind;
         stats.last_solver = RF_SOLVER_GMRES;
         if (status == -4) {  // fortran/coeffs_and_vels.f90:331-335
             rc = solve_lu(&be);
@@ -891,10 +921,11 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     if (num_fibers < 1 || P.num_terms < 1 || P.num_terms > kMaxTerms || P.num_quad_intervals < 1 ||
         3 * P.num_quad_intervals > kMaxQuad || !(P.slenderness > 0.0) || P.gmres_restart < 1 ||
         P.gmres_restart > 60 || P.gmres_max_iters < 1 || (P.precision != RF_PREC_FP64 && P.precision != RF_PREC_MIXED) ||
-        P.precond < 0 || P.precond > 2 || P.precond_neighbours < 0 || P.precond_neighbours > 6 ||
-        (P.precond == RF_PRECOND_NEIGHBOUR && num_fibers < 2)) {
+        P.precond < 0 || P.precond > 3 || P.precond_neighbours < 0 || P.precond_neighbours > 6 ||
+        P.precond_switch_iters < 0 ||
+        ((P.precond == RF_PRECOND_NEIGHBOUR || P.precond == RF_PRECOND_AUTO) && num_fibers < 2)) {
         set_error("rf_create: invalid parameters (need N>=1, 1<=M<=8, 1<=NoQI<=32, eps>0, 1<=restart<=60, "
-                  "precond 0|1|2, 0<=precond_neighbours<=6, N>=2 for the neighbour preconditioner)");
+                  "precond 0..3, 0<=precond_neighbours<=6, N>=2 for the neighbour preconditioner)");
         return RF_EINVAL;
     }
     int dev = 0, ndev = 0;
@@ -910,7 +941,8 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     s->n = 3 * s->M * s->N;
     s->mixed = (P.precision == RF_PREC_MIXED) ? 1 : 0;
     s->restart = P.gmres_restart;
-    s->ras_k = (P.precond == RF_PRECOND_NEIGHBOUR) ? std::min(num_fibers - 1, P.precond_neighbours > 0 ? P.precond_neighbours : 4) : 0;
+    s->lu_es = (s->mixed || P.lu_factor_fp32) ? 4 : 8;
+    s->ras_k = (P.precond == RF_PRECOND_NEIGHBOUR || P.precond == RF_PRECOND_AUTO) ? std::min(num_fibers - 1, P.precond_neighbours > 0 ? P.precond_neighbours : 4) : 0;
     // neighbour lists keep the 16 nearest fibers (the network's input); the
     // preconditioner uses the first ras_k of them
     s->knn = std::min(num_fibers - 1, kMlpNeighbours);

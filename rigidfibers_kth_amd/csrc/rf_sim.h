@@ -70,6 +70,8 @@ struct rf_sim {
     double* d_t = nullptr;
     int* d_pinfo = nullptr;
     bool prec_ready = false;
+    int prec_kind = 0;       // RF_PRECOND_NEIGHBOUR or RF_PRECOND_LU once prec_ready
+    int lu_es = 8;           // element bytes of the LU factors
     bool nbr_valid = false;  // neighbour lists match the current configuration
     // learned initial guess (allocated on first use)
     float* d_mlp_w = nullptr;    // weight blob
@@ -78,6 +80,7 @@ struct rf_sim {
     float* d_mlp_work = nullptr; // activations for N rows
     rf::MlpBuffers mlp_buf;
     bool mlp_weights = false;    // a trained network is loaded
+    bool r0_ready = false;       // d_r holds b - A x for the next GMRES start
     // LU workspace (allocated on first use)
     void* d_LU = nullptr;
     int* d_ipiv = nullptr;
@@ -110,14 +113,14 @@ struct rf_sim {
     int gemv_A(const double* x, double* y, const double* b, int mode);
     int fetch_dots(int count);
     int norm2(const double* v, double* out);
-    int solve_gmres(int* iters, double* be, int* status);
+    int solve_gmres(int* iters, double* be, int* status, int maxit);
     int solve_lu(double* be);
     int factorize(bool* done);
     int setup_neighbours();
     int mlp_alloc();
     int mlp_infer(double* x_out);
     int learned_guess();
-    int setup_precond();
+    int setup_precond(int kind);
     int precond_apply(const double* r, double* z);
     int update_iterate(int k);
     int phase_begin();

@@ -147,3 +147,49 @@ def test_lu_preconditioned_gmres(precision):
     assert it1 <= (3 if precision == rf.PREC_FP64 else 20), it1
     assert it1 * 10 <= it0, (it0, it1)
     assert np.linalg.norm(x1 - x0) <= 1e-8 * np.linalg.norm(x0)
+
+
+def test_auto_preconditioner_switches_to_lu_on_hard_systems():
+    """RF_PRECOND_AUTO: the neighbour preconditioner within the iteration
+    budget, then GMRES on A (LU)^-1 from the iterate reached; the result is
+    the reference GMRES solution of the same operator."""
+    pos, ori = _hard_state()
+    res = {}
+    for pc, extra in ((rf.PRECOND_NONE, {}), (rf.PRECOND_AUTO, dict(precond_switch_iters=5)),
+                      (rf.PRECOND_AUTO, dict(precond_switch_iters=2000))):
+        with _sim(pos, ori, precond=pc, **extra) as sim:
+            sim.assemble_system()
+            sim.solve_system()
+            st = sim.stats()
+            res[(pc, extra.get("precond_switch_iters"))] = (sim.coefficients(), st.last_gmres_iters,
+                                                            st.last_precond, st.last_backward_error,
+                                                            st.lu_factorizations)
+    x0 = res[(rf.PRECOND_NONE, None)][0]
+    xs, its, kind, be, nlu = res[(rf.PRECOND_AUTO, 5)]
+    assert kind == rf.PRECOND_LU and nlu == 1 and be <= 1e-10 and its <= 5 + 10
+    assert np.linalg.norm(xs - x0) <= 1e-8 * np.linalg.norm(x0)
+    xl, its, kind, be, nlu = res[(rf.PRECOND_AUTO, 2000)]
+    assert kind == rf.PRECOND_NEIGHBOUR and nlu == 0 and be <= 1e-10
+    assert np.linalg.norm(xl - x0) <= 1e-8 * np.linalg.norm(x0)
+
+
+def test_fp32_factors_of_fp64_matrix():
+    """lu_factor_fp32: FP32 LU of the FP64 matrix as the direct solver (with
+    FP64 refinement) and as the GMRES preconditioner."""
+    pos, ori = _hard_state()
+    with _sim(pos, ori, use_direct_solver=1) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        x64 = sim.coefficients()
+    with _sim(pos, ori, use_direct_solver=1, lu_factor_fp32=1, mixed_inner_tol=1e-12) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        xr, st = sim.coefficients(), sim.stats()
+    assert st.last_backward_error <= 1e-12
+    assert np.linalg.norm(xr - x64) <= 1e-9 * np.linalg.norm(x64)
+    with _sim(pos, ori, precond=rf.PRECOND_LU, lu_factor_fp32=1, gmres_tol=1e-12) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        xg, st = sim.coefficients(), sim.stats()
+    assert st.last_backward_error <= 1e-12 and st.last_gmres_iters <= 20
+    assert np.linalg.norm(xg - x64) <= 1e-9 * np.linalg.norm(x64)

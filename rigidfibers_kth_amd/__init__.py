@@ -9,8 +9,8 @@ reference's host interface:
 * ``Simulation(configuration)`` / ``Simulation.step()`` --
   src/simulation.h:36-87, src/simulation.cu:279-301.
 
-There is no CPU fallback: every call goes to the HIP library, and importing
-this package raises if the library has not been built.
+There is no CPU fallback: every call goes to the HIP library, and the first
+call raises if the library has not been built.
 """
 import ctypes
 import os

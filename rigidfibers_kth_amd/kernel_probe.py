@@ -1,0 +1,78 @@
+"""Runs one kernel family of the timestep in isolation, for profilers.
+
+    python3 -m rigidfibers_kth_amd.kernel_probe gemv --fibers 1000 --reps 20
+    python3 -m rigidfibers_kth_amd.kernel_probe copy --bytes 1073741824 --reps 5
+    python3 -m rigidfibers_kth_amd.kernel_probe lu --fibers 1000 [--fp32]
+    python3 -m rigidfibers_kth_amd.kernel_probe assemble --fibers 1000 --reps 5
+
+The layouts are the seeded ones of the benchmark (volume fraction 1e-2 for
+N <= 2000, 5e-2 above).  Each run prints what it launched, so a profile of
+the command can be read against it: `gemv` launches the system-matrix GEMV
+(rf_matvec) `reps` times after one assembly, `copy` streams a buffer through
+the plain copy kernel (HBM byte-counter calibration), `lu` factorises the
+assembled matrix once, `assemble` assembles `reps` times.
+"""
+import argparse
+import sys
+import time
+
+import numpy as np
+
+from . import PREC_FP64, PREC_MIXED, Configuration, Simulation, default_params, selftest_copy
+from .layouts import layout
+
+
+def _sim(fibers, **kw):
+    phi, seed = (1e-2, 0x5EED0001) if fibers <= 2000 else (5e-2, 0x5EED0003)
+    pos, ori = layout(fibers, phi, seed)
+    return Simulation(Configuration(default_params(**kw), pos, ori))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("kernel", choices=["gemv", "copy", "lu", "assemble"])
+    ap.add_argument("--fibers", type=int, default=1000)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--bytes", type=int, default=1 << 30)
+    ap.add_argument("--fp32", action="store_true", help="FP32 matrix (gemv, assemble) or FP32 factors (lu)")
+    a = ap.parse_args(argv)
+    if a.kernel == "copy":
+        ms = selftest_copy(a.bytes, a.reps)
+        print("copy: %d bytes x %d launches, %.3f ms each, %.1f GB/s (read + write)"
+              % (a.bytes, a.reps, ms, 2 * a.bytes / (ms * 1e-3) / 1e9), flush=True)
+        return 0
+    prec = PREC_MIXED if (a.fp32 and a.kernel != "lu") else PREC_FP64
+    kw = dict(precision=prec, profile=1)
+    if a.kernel == "lu":
+        kw.update(use_direct_solver=1, lu_factor_fp32=int(a.fp32))
+    with _sim(a.fibers, **kw) as sim:
+        sim.assemble_system()
+        sim.synchronize()
+        if a.kernel == "assemble":
+            for _ in range(a.reps - 1):
+                sim.assemble_system()
+            sim.synchronize()
+            st = sim.stats()
+            print("assemble: N=%d, %d launches, %.3f ms each" % (a.fibers, st.assemble_launches,
+                                                                 st.assemble_ms_total / st.assemble_launches))
+        elif a.kernel == "gemv":
+            x = np.random.default_rng(0).normal(size=sim.n)
+            t0 = time.perf_counter()
+            for _ in range(a.reps):
+                sim.matvec(x)
+            sim.synchronize()
+            st = sim.stats()
+            print("gemv: n=%d, element bytes %d, %d launches, %.3f ms each (events), %.3f ms wall each"
+                  % (sim.n, st.matrix_elem_bytes, st.gemv_launches, st.gemv_ms_total / max(1, st.gemv_launches),
+                     1e3 * (time.perf_counter() - t0) / a.reps), flush=True)
+        else:
+            sim.solve_system()
+            sim.synchronize()
+            st = sim.stats()
+            print("lu: n=%d, factors %s, %.1f ms, backward error %.2e"
+                  % (sim.n, "fp32" if a.fp32 else "fp64", st.lu_ms_total, st.last_backward_error), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -9,7 +9,8 @@ namespace rf {
 
 // In-place LU of the n x n column-major matrix (double if elem_bytes == 8,
 // float otherwise); ipiv[k] = row swapped with k (0-based); *info = k+1 if
-// U(k,k) == 0.  use_mfma selects the matrix-core trailing update.  ws:
+// U(k,k) == 0.  use_mfma selects the matrix-core trailing update; panel
+// width nb <= 128 for float, <= 64 for double.  ws:
 // lu_workspace_bytes() of device memory whose last word (the column
 // kernels' counter) is zero on entry and is left zero.
 size_t lu_workspace_bytes();

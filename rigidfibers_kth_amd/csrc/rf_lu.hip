@@ -26,6 +26,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kColThreads = 256;
 constexpr int kColMaxBlocks = 512;
+constexpr int kMaxPanel = 128;  // panel width: 128 for FP32 factors, 64 for FP64
 
 // (value, index) maximum with ties to the smaller index (LAPACK idamax keeps
 // the first maximal entry) over one workgroup; result valid in thread 0.
@@ -65,8 +66,9 @@ __device__ __forceinline__ void block_argmax(double& best, int& bi, double* sv, 
 //    and stores it in column j,
 //  * and the last workgroup to finish (atomic counter) picks the pivot of
 //    column j (first largest |v_i|, LAPACK idamax), records ipiv[j], swaps
-//    rows j and p over the panel columns, and computes U[k0..j+1, j+1] of the
-//    next column by forward substitution with the unit lower L11.
+//    rows j and p over the panel columns, and forms row j of U for the panel
+//    columns right of j (one short dot product per column), so that the next
+//    launch finds U[k0..j+1, j+1] complete.
 // Only two columns are written per launch (the right-looking form rewrites
 // the whole rest of the panel every column).
 template <typename T>
@@ -74,9 +76,8 @@ __global__ void __launch_bounds__(kColThreads) lu_crout_kernel(T* A, int64_t lda
                                                                int* ipiv, int* info, double* pval, int* pidx,
                                                                unsigned int* counter)
 {
-    __shared__ double sU[64];
-    __shared__ double sL[64][65];  // L11 rows for the next column's U
-    __shared__ double sB[64];
+    __shared__ double sU[kMaxPanel];   // U[k0..j, j]
+    __shared__ double sLr[kMaxPanel];  // row j of L (after the swap)
     __shared__ double sv[kColThreads / 64];
     __shared__ int si[kColThreads / 64];
     __shared__ int last_s, piv_s;
@@ -159,28 +160,16 @@ __global__ void __launch_bounds__(kColThreads) lu_crout_kernel(T* A, int64_t lda
         }
     }
     __syncthreads();
-    // U[k0..j+1, j+1]: r = k0..j, U[r] = A[r, j+1] - sum_{q < r} L[r, q] U[q].
-    // L11 rows and the right-hand side are read into LDS with all threads at
-    // once; one wavefront then runs the substitution (lane q keeps U[k0 + q]).
+    // Row j of U for the panel columns right of j (rows above j were done by
+    // the earlier launches): U[j,c] = A[j,c] - sum_{q in [k0,j)} L[j,q] U[q,c].
     if (j + 1 >= k1) return;
-    T* cn = A + (int64_t)(j + 1) * lda;
-    for (int e = threadIdx.x; e < (m + 1) * m; e += kColThreads) {
-        const int r = e % (m + 1), q = e / (m + 1);
-        sL[r][q] = (double)A[(int64_t)(k0 + q) * lda + k0 + r];
-    }
-    for (int r = threadIdx.x; r <= m; r += kColThreads) sB[r] = (double)cn[k0 + r];
+    for (int q = threadIdx.x; q < m; q += kColThreads) sLr[q] = (double)A[(int64_t)(k0 + q) * lda + j];
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        double u = 0.0;
-        for (int r = 0; r <= m; ++r) {
-            double t = (lane < r) ? sL[r][lane] * u : 0.0;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-            const double ur = sB[r] - t;
-            if (lane == r) u = ur;
-        }
-        if (lane <= m) cn[k0 + lane] = (T)u;
+    for (int c = j + 1 + threadIdx.x; c < k1; c += kColThreads) {
+        T* cc = A + (int64_t)c * lda;
+        double u = (double)cc[j];
+        for (int q = 0; q < m; ++q) u -= sLr[q] * (double)cc[k0 + q];
+        cc[j] = (T)u;
     }
 }
 
@@ -222,33 +211,35 @@ __global__ void lu_laswp_kernel(T* A, int64_t lda, int n, int k0, int k1, const 
 // LDS against L11 (also in LDS).
 constexpr int kTrsmCols = 128;
 
+// LDS holds L11 and the slab in the factors' precision (FP32 factors use a
+// 128-wide panel: 2 x 66 KB).
 template <typename T, int NB>
 __global__ void __launch_bounds__(kTrsmCols) lu_trsm_kernel(T* A, int64_t lda, int n, int k0, int k1)
 {
-    __shared__ double L[NB][NB + 1];
-    __shared__ double S[NB][kTrsmCols + 1];
+    __shared__ T L[NB][NB + 1];
+    __shared__ T S[NB][kTrsmCols + 1];
     const int nb = k1 - k0;
     for (int e = threadIdx.x; e < nb * nb; e += kTrsmCols) {
         const int r = e % nb, c = e / nb;
-        L[r][c] = (double)A[(int64_t)(k0 + c) * lda + k0 + r];
+        L[r][c] = A[(int64_t)(k0 + c) * lda + k0 + r];
     }
     const int cbase = k1 + blockIdx.x * kTrsmCols;
     for (int e = threadIdx.x; e < nb * kTrsmCols; e += kTrsmCols) {
         const int r = e % nb, c = e / nb;
         const int col = cbase + c;
-        S[r][c] = (col < n) ? (double)A[(int64_t)col * lda + k0 + r] : 0.0;
+        S[r][c] = (col < n) ? A[(int64_t)col * lda + k0 + r] : T(0);
     }
     __syncthreads();
     const int c = threadIdx.x;
     for (int r = 0; r < nb; ++r) {
-        const double xr = S[r][c];
+        const T xr = S[r][c];
         for (int q = r + 1; q < nb; ++q) S[q][c] -= L[q][r] * xr;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nb * kTrsmCols; e += kTrsmCols) {
         const int r = e % nb, cc = e / nb;
         const int col = cbase + cc;
-        if (col < n) A[(int64_t)col * lda + k0 + r] = (T)S[r][cc];
+        if (col < n) A[(int64_t)col * lda + k0 + r] = S[r][cc];
     }
 }
 
@@ -362,11 +353,13 @@ struct GemmStage {
             const int kk = v / (kBM / VL), r0 = (v % (kBM / VL)) * VL;
             const int row = rbase + r0;
             const T* src = A + (int64_t)(kc + kk) * lda + row;
+            // FP64: -L21, the accumulators start at C (see the kernel)
+            constexpr T sgn = (sizeof(T) == 8) ? T(-1) : T(1);
             if (vec && kk < kcount && row + VL <= n) {
-                a[t] = *reinterpret_cast<const vec_t*>(src);
+                a[t] = sgn * *reinterpret_cast<const vec_t*>(src);
             } else {
 #pragma unroll
-                for (int u = 0; u < VL; ++u) a[t][u] = (kk < kcount && row + u < n) ? src[u] : T(0);
+                for (int u = 0; u < VL; ++u) a[t][u] = (kk < kcount && row + u < n) ? sgn * src[u] : T(0);
             }
         }
 #pragma unroll
@@ -413,11 +406,26 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, in
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave >> 1) * 64, wc = (wave & 1) * (BN / 2);
     const int lr = lane & 15, lk = lane >> 4;
+    // FP64: the accumulators start as the C tile (loads issued ahead of the
+    // K loop) and the MFMAs add (-L21) U12, hiding the tile's read latency
+    // behind the first chunk.  FP32 keeps zero accumulators and subtracts at
+    // the end: preloading C there costs a wavefront per SIMD of occupancy.
     V4 acc[4][JB];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < JB; ++j) acc[i][j] = V4{0, 0, 0, 0};
+        for (int j = 0; j < JB; ++j) {
+            if constexpr (kF64) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = rbase + wr + 16 * i + lr;
+                    const int col = cbase + wc + 16 * j + lk + 4 * r;
+                    acc[i][j][r] = (row < n && col < n) ? A[(int64_t)col * lda + row] : T(0);
+                }
+            } else {
+                acc[i][j] = V4{0, 0, 0, 0};
+            }
+        }
     GemmStage<T> st;
     // 16-byte vectors need rows and K offsets that are multiples of the vector
     const bool vec = (k0 % GemmStage<T>::VL == 0) && (k1 % GemmStage<T>::VL == 0) && (lda % GemmStage<T>::VL == 0);
@@ -454,7 +462,7 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, in
                 for (int r = 0; r < 4; ++r) {
                     const int row = rbase + wr + 16 * i + lr;
                     const int col = cbase + wc + 16 * j + lk + 4 * r;
-                    if (row < n && col < n) A[(int64_t)col * lda + row] -= acc[i][j][r];
+                    if (row < n && col < n) A[(int64_t)col * lda + row] = acc[i][j][r];
                 }
             } else {
                 const int row0 = rbase + wr + 16 * i + 4 * lk;
@@ -551,8 +559,16 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
             hipLaunchKernelGGL(lu_laswp_kernel<T>, dim3((outside + 255) / 256), dim3(256), 0, s, A, lda, n, k0, k1,
                                ipiv);
         if (k1 < n) {
-            hipLaunchKernelGGL((lu_trsm_kernel<T, 64>), dim3((n - k1 + kTrsmCols - 1) / kTrsmCols), dim3(kTrsmCols),
-                               0, s, A, lda, n, k0, k1);
+            const dim3 tgrid((n - k1 + kTrsmCols - 1) / kTrsmCols);
+            bool wide = false;
+            if constexpr (sizeof(T) == 4) {
+                if (nb > 64) {
+                    hipLaunchKernelGGL((lu_trsm_kernel<T, kMaxPanel>), tgrid, dim3(kTrsmCols), 0, s, A, lda, n, k0, k1);
+                    wide = true;
+                }
+            }
+            if (!wide)
+                hipLaunchKernelGGL((lu_trsm_kernel<T, 64>), tgrid, dim3(kTrsmCols), 0, s, A, lda, n, k0, k1);
             if (use_mfma) {
                 const int rem = n - k1, bn = gemm_bn<T>();
                 hipLaunchKernelGGL(lu_gemm_mfma_kernel<T>, dim3((rem + kBM - 1) / kBM, (rem + bn - 1) / bn), dim3(256),
@@ -633,7 +649,7 @@ size_t lu_workspace_bytes() { return kColMaxBlocks * (sizeof(double) + sizeof(in
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
                      void* ws, hipStream_t s)
 {
-    if (nb < 1 || nb > 64 || !ws || lda % 4 != 0) return hipErrorInvalidValue;
+    if (nb < 1 || nb > (elem_bytes == 4 ? kMaxPanel : 64) || !ws || lda % 4 != 0) return hipErrorInvalidValue;
     if (elem_bytes == 8)
         return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, ws, s);
     return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, ws, s);

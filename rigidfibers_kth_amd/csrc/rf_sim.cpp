@@ -220,7 +220,12 @@ void rf_sim::timed_push(hipEvent_t a, hipEvent_t b, int kind) { ev_pending.push_
 // Call only when the stream is idle (after a synchronisation).
 void rf_sim::resolve_timed()
 {
+    std::vector<Timed> running;  // pairs whose end event has not completed yet
     for (const Timed& t : ev_pending) {
+        if (hipEventQuery(t.b) != hipSuccess) {
+            running.push_back(t);
+            continue;
+        }
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) ms = 0.f;
         switch (t.kind) {
@@ -242,7 +247,7 @@ void rf_sim::resolve_timed()
         }
         ev_free.push_back({t.a, t.b});
     }
-    ev_pending.clear();
+    ev_pending.swap(running);
 }
 
 int rf_sim::gemv_A(const double* x, double* y, const double* b, int mode)
@@ -332,6 +337,9 @@ int rf_sim::update_iterate(int k)
 // the same iterates as the reference's modified Gram-Schmidt.  With
 // rf_params.precond the same iteration runs on A M^-1 (right preconditioning,
 // the stopping test still uses the true residual b - A x).
+// iterations done after step k of restart cycle `cycle`
+static inline int total_before(int cycle, int m, int k) { return cycle * m + k; }
+
 int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int maxit)
 {
     const int m = restart;
@@ -369,27 +377,39 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
     double* D1 = d_dots;
     double* D2 = d_dots + (m + 2);
     double* D3 = d_dots + 2 * (m + 2);
+    // w = A v_j, or A M^-1 v_j with the right preconditioner
+    auto matvec = [&](int j) -> int {
+        const double* vj = d_V + (size_t)j * lda;
+        if (!prec_ready) return gemv_A(vj, d_w, nullptr, 0);
+        const int r = precond_apply(vj, d_z);
+        return r ? r : gemv_A(d_z, d_w, nullptr, 0);
+    };
     for (;;) {
         std::fill(H.begin(), H.end(), 0.0);
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         double bea = 0.0;
         bool have_xc = false;
+        bool w_ready = false;  // d_w already holds the product for v_jH
         for (int jH = 0; jH < m; ++jH) {
             const int k = jH + 1;
-            double* vj = d_V + (size_t)jH * ldv;
-            if (prec_ready) {  // w = A M^-1 v_j
-                if ((rc = precond_apply(vj, d_z))) return rc;
-                if ((rc = gemv_A(d_z, d_w, nullptr, 0))) return rc;
-            } else {
-                if ((rc = gemv_A(vj, d_w, nullptr, 0))) return rc;
-            }
+            if (!w_ready && (rc = matvec(jH))) return rc;
+            w_ready = false;
             HIP_TRY(launch_multidot(d_V, ldv, k, d_w, n, d_dparts, d_counter, D1, stream));
             HIP_TRY(launch_orth_update(d_V, ldv, k, D1, d_w, n, 1, d_dparts, d_counter, D2, stream));
             HIP_TRY(launch_orth_update(d_V, ldv, k, D2, d_w, n, 0, nullptr, nullptr, nullptr, stream));
             HIP_TRY(launch_multidot(nullptr, 0, 0, d_w, n, d_dparts, d_counter, D3, stream));
             if (k < m) HIP_TRY(launch_scale_copy(d_w, D3, 0, n, d_V + (size_t)k * ldv, stream));
-            if ((rc = fetch_dots(3 * (m + 2)))) return rc;
+            // The Hessenberg column goes to the host; the next basis vector is
+            // already on the device, so its product is queued before the host
+            // waits (it is discarded if this iteration ends the solve).
+            HIP_TRY(hipMemcpyAsync(h_dots, d_dots, sizeof(double) * 3 * (m + 2), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipEventRecord(ev_k0, stream));
+            if (k < m && total_before(iterOut, m, k) < maxit) {
+                if ((rc = matvec(k))) return rc;
+                w_ready = true;
+            }
+            HIP_TRY(hipEventSynchronize(ev_k0));
             for (int j = 0; j < k; ++j) H[j + (size_t)jH * ldh] = h_dots[j] + h_dots[(m + 2) + j];
             const double dnormw = std::sqrt(h_dots[2 * (m + 2)]);
             H[k + (size_t)jH * ldh] = dnormw;
@@ -470,7 +490,9 @@ int rf_sim::factorize(bool* done)
     else  // FP32 factors of the FP64 matrix
         HIP_TRY(launch_f64_to_f32(static_cast<const double*>(d_A), static_cast<float*>(d_LU), (size_t)lda * n, stream));
     HIP_TRY(hipMemsetAsync(d_info, 0, sizeof(int), stream));
-    const int nb = (p.lu_block > 0) ? std::min(64, p.lu_block) : 64;
+    // panel width: 128 for FP32 factors (halves the trailing update's traffic), 64 for FP64
+    const int nb_max = (es == 4) ? 128 : 64;
+    const int nb = (p.lu_block > 0) ? std::min(nb_max, p.lu_block) : nb_max;
     hipEvent_t ea = nullptr, eb = nullptr;
     int rc0;
     if (p.profile) {

@@ -173,21 +173,23 @@ def test_auto_preconditioner_switches_to_lu_on_hard_systems():
     assert np.linalg.norm(xl - x0) <= 1e-8 * np.linalg.norm(x0)
 
 
-def test_fp32_factors_of_fp64_matrix():
-    """lu_factor_fp32: FP32 LU of the FP64 matrix as the direct solver (with
-    FP64 refinement) and as the GMRES preconditioner."""
+@pytest.mark.parametrize("nb", [0, 100, 24])
+def test_fp32_factors_of_fp64_matrix(nb):
+    """lu_factor_fp32: FP32 LU of the FP64 matrix (panel width nb, 0 = 128)
+    as the direct solver (with FP64 refinement) and as the GMRES
+    preconditioner."""
     pos, ori = _hard_state()
     with _sim(pos, ori, use_direct_solver=1) as sim:
         sim.assemble_system()
         sim.solve_system()
         x64 = sim.coefficients()
-    with _sim(pos, ori, use_direct_solver=1, lu_factor_fp32=1, mixed_inner_tol=1e-12) as sim:
+    with _sim(pos, ori, use_direct_solver=1, lu_factor_fp32=1, mixed_inner_tol=1e-12, lu_block=nb) as sim:
         sim.assemble_system()
         sim.solve_system()
         xr, st = sim.coefficients(), sim.stats()
     assert st.last_backward_error <= 1e-12
     assert np.linalg.norm(xr - x64) <= 1e-9 * np.linalg.norm(x64)
-    with _sim(pos, ori, precond=rf.PRECOND_LU, lu_factor_fp32=1, gmres_tol=1e-12) as sim:
+    with _sim(pos, ori, precond=rf.PRECOND_LU, lu_factor_fp32=1, gmres_tol=1e-12, lu_block=nb) as sim:
         sim.assemble_system()
         sim.solve_system()
         xg, st = sim.coefficients(), sim.stats()

@@ -96,6 +96,7 @@ typedef struct rf_params {
     int32_t precond_switch_iters; /* RF_PRECOND_AUTO: neighbour-preconditioned iterations before the LU (0 = 2 restarts) */
     int32_t lu_factor_fp32;       /* 1: FP32 LU factors also for the FP64 matrix (preconditioner / refinement) */
     int32_t reserved[3];
+    double refine_tol;            /* GMRES tolerance once the LU preconditioner is in use (0 = gmres_tol) */
 } rf_params;
 
 typedef struct rf_stats {

@@ -58,6 +58,7 @@ class rf_params(ctypes.Structure):
         ("precond_switch_iters", ctypes.c_int32),
         ("lu_factor_fp32", ctypes.c_int32),
         ("reserved", ctypes.c_int32 * 3),
+        ("refine_tol", ctypes.c_double),
     ]
 
 

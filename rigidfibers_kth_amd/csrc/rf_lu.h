@@ -16,8 +16,19 @@ namespace rf {
 size_t lu_workspace_bytes();
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
                      void* ws, hipStream_t s);
-// x <- U^-1 L^-1 P x with FP64 vectors.
+// x <- U^-1 L^-1 P x with FP64 vectors (substitution, two launches per
+// 64-row block).
 hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s);
+
+// The same solve with one launch per 64-row block and sweep, for the factors
+// that precondition every GMRES iteration.  aux holds lu_solve_aux_bytes(n):
+// the row permutation, filled by lu_prepare_solve once per factorisation;
+// tmp is n doubles.
+size_t lu_solve_aux_bytes(int n);
+hipError_t lu_prepare_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, void* aux,
+                            hipStream_t s);
+hipError_t lu_solve_blocks(const void* LU, int elem_bytes, int64_t lda, int n, const void* aux, double* x,
+                           double* tmp, hipStream_t s);
 
 // Lane-layout probe of the 16x16x4 matrix-core instructions (one wavefront).
 hipError_t mfma_probe(int mode, double* out64, float* out32, hipStream_t s);

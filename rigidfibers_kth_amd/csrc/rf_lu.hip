@@ -636,6 +636,81 @@ __global__ void mfma_probe_kernel(int mode, double* out64, float* out32)
     }
 }
 
+// ---- solves with FP32 factors: permutation + one launch per block ----------
+// A solve with the factors of the mixed mode runs once per GMRES iteration,
+// so it is built for few launches: the pivots become one permutation (once
+// per factorisation) and each 64-row block of the forward and backward sweeps
+// is one launch.  Every workgroup of the launch reads the block's 64x64
+// diagonal triangle into LDS and solves it by substitution with one wavefront
+// (lane r keeps x_r, the solved entry is broadcast with a lane shuffle), then
+// updates its share of the remaining rows; workgroup 0 stores the block's
+// solution.
+constexpr int kSB = 64;
+
+__global__ void pivots_to_perm_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ perm)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        const int p = ipiv[k];
+        if (p != k) {
+            const int t = perm[k];
+            perm[k] = perm[p];
+            perm[p] = t;
+        }
+    }
+}
+
+__global__ void permute_kernel(const double* __restrict__ x, const int* __restrict__ perm, int n,
+                               double* __restrict__ y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[perm[i]];
+}
+
+// lower = 1: forward block (unit lower L), rows updated i >= k0+bs;
+// lower = 0: backward block (U), rows updated i < k0.
+template <typename T, int LOWER>
+__global__ void __launch_bounds__(256) sweep_block_kernel(const T* __restrict__ LU, int64_t lda, int n,
+                                                          double* __restrict__ v, double* __restrict__ out, int k0,
+                                                          int bs)
+{
+    __shared__ double D[kSB][kSB + 1];  // D[r][q] = A[k0+r, k0+q]
+    __shared__ double yk[kSB];
+    const int t = threadIdx.x;
+    for (int e = t; e < bs * bs; e += 256) {
+        const int r = e % bs, q = e / bs;
+        D[r][q] = (double)LU[(int64_t)(k0 + q) * lda + k0 + r];
+    }
+    __syncthreads();
+    if (t < 64) {
+        double x = (t < bs) ? v[k0 + t] : 0.0;
+        if (LOWER) {
+            for (int q = 0; q < bs; ++q) {
+                const double xq = __shfl(x, q, 64);
+                if (t > q && t < bs) x -= D[t][q] * xq;
+            }
+        } else {
+            for (int q = bs - 1; q >= 0; --q) {
+                if (t == q) x /= D[q][q];
+                const double xq = __shfl(x, q, 64);
+                if (t < q) x -= D[t][q] * xq;
+            }
+        }
+        if (t < bs) {
+            yk[t] = x;
+            if (blockIdx.x == 0) out[k0 + t] = x;
+        }
+    }
+    __syncthreads();
+    const int i = LOWER ? k0 + bs + blockIdx.x * 256 + t : blockIdx.x * 256 + t;
+    if (LOWER ? (i < n) : (i < k0)) {
+        double acc = v[i];
+        for (int q = 0; q < bs; ++q) acc -= (double)LU[(int64_t)(k0 + q) * lda + i] * yk[q];
+        v[i] = acc;
+    }
+}
+
 }  // namespace
 
 hipError_t mfma_probe(int mode, double* out64, float* out32, hipStream_t s)
@@ -653,6 +728,48 @@ hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* i
     if (elem_bytes == 8)
         return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, ws, s);
     return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, ws, s);
+}
+
+size_t lu_solve_aux_bytes(int n) { return ((sizeof(int) * (size_t)n + 255) / 256) * 256; }
+
+hipError_t lu_prepare_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, void* aux,
+                            hipStream_t s)
+{
+    (void)LU;
+    (void)elem_bytes;
+    (void)lda;
+    hipLaunchKernelGGL(pivots_to_perm_kernel, dim3(1), dim3(64), 0, s, ipiv, n, static_cast<int*>(aux));
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t solve_blocks_impl(const T* LU, int64_t lda, int n, const void* aux, double* x, double* tmp, hipStream_t s)
+{
+    const int* perm = static_cast<const int*>(aux);
+    const int nb = (n + kSB - 1) / kSB;
+    hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, perm, n, tmp);
+    for (int b = 0; b < nb; ++b) {  // L z = P x: tmp -> x
+        const int k0 = b * kSB, bs = std::min(kSB, n - k0);
+        const int rows = n - k0 - bs;
+        hipLaunchKernelGGL((sweep_block_kernel<T, 1>), dim3(std::max(1, (rows + 255) / 256)), dim3(256), 0, s, LU,
+                           lda, n, tmp, x, k0, bs);
+    }
+    for (int b = nb - 1; b >= 0; --b) {  // U y = z: x -> tmp
+        const int k0 = b * kSB, bs = std::min(kSB, n - k0);
+        hipLaunchKernelGGL((sweep_block_kernel<T, 0>), dim3(std::max(1, (k0 + 255) / 256)), dim3(256), 0, s, LU,
+                           lda, n, x, tmp, k0, bs);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(x, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+}
+
+hipError_t lu_solve_blocks(const void* LU, int elem_bytes, int64_t lda, int n, const void* aux, double* x,
+                           double* tmp, hipStream_t s)
+{
+    if (elem_bytes == 8)
+        return solve_blocks_impl<double>(static_cast<const double*>(LU), lda, n, aux, x, tmp, s);
+    return solve_blocks_impl<float>(static_cast<const float*>(LU), lda, n, aux, x, tmp, s);
 }
 
 hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s)

@@ -343,7 +343,8 @@ static inline int total_before(int cycle, int m, int k) { return cycle * m + k; 
 int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int maxit)
 {
     const int m = restart;
-    const double tol = (mixed && p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : p.gmres_tol;
+    double tol = (mixed && p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : p.gmres_tol;
+    if (prec_ready && prec_kind == RF_PRECOND_LU && p.refine_tol > 0.0) tol = p.refine_tol;
     const int ldh = m + 1;
     std::vector<double> H((size_t)(m + 1) * (m + 2), 0.0), g(m + 1, 0.0), cs(m + 1), sn(m + 1), y(m + 1);
     int rc;
@@ -514,7 +515,30 @@ int rf_sim::factorize(bool* done)
         rf::set_error("LU: matrix is singular (U(" + std::to_string(info) + "," + std::to_string(info) + ") = 0)");
         return RF_ENOCONV;
     }
+    if (es == 4) {  // FP32 factors are applied every iteration: permutation + block inverses
+        if (!d_lu_aux) {
+            void* q = nullptr;
+            if (hipMalloc(&q, lu_solve_aux_bytes(n) + sizeof(double) * (size_t)lda) != hipSuccess) {
+                rf::set_error("cannot allocate the LU solve workspace");
+                return RF_ENOMEM;
+            }
+            allocs.push_back(q);
+            d_lu_aux = q;
+            d_lu_tmp = reinterpret_cast<double*>(static_cast<char*>(q) + lu_solve_aux_bytes(n));
+        }
+        HIP_TRY(lu_prepare_solve(d_LU, es, lda, n, d_ipiv, d_lu_aux, stream));
+    }
     *done = true;
+    return RF_OK;
+}
+
+// x <- (LU)^-1 x with the current factors
+int rf_sim::lu_apply(double* x)
+{
+    if (lu_es == 4)
+        HIP_TRY(lu_solve_blocks(d_LU, lu_es, lda, n, d_lu_aux, x, d_lu_tmp, stream));
+    else
+        HIP_TRY(lu_solve(d_LU, lu_es, lda, n, d_ipiv, x, stream));
     return RF_OK;
 }
 
@@ -535,7 +559,7 @@ int rf_sim::solve_lu(double* be_out)
     const double bn = std::sqrt(bn2);
     if (es == 8) {
         HIP_TRY(hipMemcpyAsync(d_x, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-        HIP_TRY(lu_solve(d_LU, es, lda, n, d_ipiv, d_x, stream));
+        if ((rc = lu_apply(d_x))) return rc;
         if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
         double rr;
         if ((rc = norm2(d_r, &rr))) return rc;
@@ -548,7 +572,7 @@ int rf_sim::solve_lu(double* be_out)
     HIP_TRY(hipMemcpyAsync(d_r, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
     double be = 1.0;
     for (int it = 0; it < 30; ++it) {
-        HIP_TRY(lu_solve(d_LU, es, lda, n, d_ipiv, d_r, stream));
+        if ((rc = lu_apply(d_r))) return rc;
         HIP_TRY(launch_axpby(n, 1.0, d_x, 1.0, d_r, d_x, stream));
         if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
         double rr;
@@ -738,8 +762,7 @@ int rf_sim::precond_apply(const double* r, double* z)
 {
     if (prec_kind == RF_PRECOND_LU) {
         if (r != z) HIP_TRY(hipMemcpyAsync(z, r, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-        HIP_TRY(lu_solve(d_LU, lu_es, lda, n, d_ipiv, z, stream));
-        return RF_OK;
+        return lu_apply(z);
     }
     HIP_TRY(launch_ras_apply(N, ras_k, knn, 3 * M, d_nbr, d_B, r, z, stream));
     return RF_OK;

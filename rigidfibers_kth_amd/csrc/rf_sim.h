@@ -86,6 +86,8 @@ struct rf_sim {
     int* d_ipiv = nullptr;
     int* d_info = nullptr;
     void* d_lu_ws = nullptr;   // pivot-search partials and counter
+    void* d_lu_aux = nullptr;  // FP32 factors: permutation and diagonal-block inverses
+    double* d_lu_tmp = nullptr;
     bool lu_factored = false;  // d_LU holds the factors of the current matrix
 
     rf_stats stats{};
@@ -116,6 +118,7 @@ struct rf_sim {
     int solve_gmres(int* iters, double* be, int* status, int maxit);
     int solve_lu(double* be);
     int factorize(bool* done);
+    int lu_apply(double* x);
     int setup_neighbours();
     int mlp_alloc();
     int mlp_infer(double* x_out);

@@ -168,7 +168,15 @@ __global__ void __launch_bounds__(kColThreads) lu_crout_kernel(T* A, int64_t lda
     for (int c = j + 1 + threadIdx.x; c < k1; c += kColThreads) {
         T* cc = A + (int64_t)c * lda;
         double u = (double)cc[j];
-        for (int q = 0; q < m; ++q) u -= sLr[q] * (double)cc[k0 + q];
+        int q = 0;
+        for (; q + 8 <= m; q += 8) {  // eight loads in flight
+            double a[8];
+#pragma unroll
+            for (int w = 0; w < 8; ++w) a[w] = (double)cc[k0 + q + w];
+#pragma unroll
+            for (int w = 0; w < 8; ++w) u -= sLr[q + w] * a[w];
+        }
+        for (; q < m; ++q) u -= sLr[q] * (double)cc[k0 + q];
         cc[j] = (T)u;
     }
 }
@@ -205,38 +213,38 @@ __global__ void lu_laswp_kernel(T* A, int64_t lda, int n, int k0, int k1, const 
     }
 }
 
-// U12 = L11^-1 A12: columns c in [k1, n), rows [k0, k1).  A workgroup
-// moves a 128-column slab into LDS with column-contiguous (coalesced) global
-// accesses; each thread then runs the forward substitution of its column in
-// LDS against L11 (also in LDS).
-constexpr int kTrsmCols = 128;
+// U12 = L11^-1 A12: columns c in [k1, n), rows [k0, k1).  A workgroup of
+// 256 threads takes a 32-column slab into LDS (column-contiguous, coalesced
+// reads) with L11; the forward substitution runs row by row, the update of
+// each column's remaining rows split over 8 threads, one barrier per row.
+// LDS holds both in the factors' precision (FP32 factors use a 128-wide
+// panel: 64 KB + 17 KB).
+constexpr int kTrsmC = 32, kTrsmP = 8;
 
-// LDS holds L11 and the slab in the factors' precision (FP32 factors use a
-// 128-wide panel: 2 x 66 KB).
 template <typename T, int NB>
-__global__ void __launch_bounds__(kTrsmCols) lu_trsm_kernel(T* A, int64_t lda, int n, int k0, int k1)
+__global__ void __launch_bounds__(kTrsmC * kTrsmP) lu_trsm_kernel(T* A, int64_t lda, int n, int k0, int k1)
 {
-    __shared__ T L[NB][NB + 1];
-    __shared__ T S[NB][kTrsmCols + 1];
+    __shared__ T L[NB][NB];
+    __shared__ T S[NB][kTrsmC + 1];
     const int nb = k1 - k0;
-    for (int e = threadIdx.x; e < nb * nb; e += kTrsmCols) {
+    for (int e = threadIdx.x; e < nb * nb; e += kTrsmC * kTrsmP) {
         const int r = e % nb, c = e / nb;
         L[r][c] = A[(int64_t)(k0 + c) * lda + k0 + r];
     }
-    const int cbase = k1 + blockIdx.x * kTrsmCols;
-    for (int e = threadIdx.x; e < nb * kTrsmCols; e += kTrsmCols) {
+    const int cbase = k1 + blockIdx.x * kTrsmC;
+    for (int e = threadIdx.x; e < nb * kTrsmC; e += kTrsmC * kTrsmP) {
         const int r = e % nb, c = e / nb;
         const int col = cbase + c;
         S[r][c] = (col < n) ? A[(int64_t)col * lda + k0 + r] : T(0);
     }
     __syncthreads();
-    const int c = threadIdx.x;
+    const int c = threadIdx.x % kTrsmC, part = threadIdx.x / kTrsmC;
     for (int r = 0; r < nb; ++r) {
-        const T xr = S[r][c];
-        for (int q = r + 1; q < nb; ++q) S[q][c] -= L[q][r] * xr;
+        const T xr = S[r][c];  // final: rows above r were applied in earlier steps
+        for (int q = r + 1 + part; q < nb; q += kTrsmP) S[q][c] -= L[q][r] * xr;
+        __syncthreads();
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < nb * kTrsmCols; e += kTrsmCols) {
+    for (int e = threadIdx.x; e < nb * kTrsmC; e += kTrsmC * kTrsmP) {
         const int r = e % nb, cc = e / nb;
         const int col = cbase + cc;
         if (col < n) A[(int64_t)col * lda + k0 + r] = S[r][cc];
@@ -559,16 +567,15 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
             hipLaunchKernelGGL(lu_laswp_kernel<T>, dim3((outside + 255) / 256), dim3(256), 0, s, A, lda, n, k0, k1,
                                ipiv);
         if (k1 < n) {
-            const dim3 tgrid((n - k1 + kTrsmCols - 1) / kTrsmCols);
+            const dim3 tgrid((n - k1 + kTrsmC - 1) / kTrsmC), tblock(kTrsmC * kTrsmP);
             bool wide = false;
             if constexpr (sizeof(T) == 4) {
                 if (nb > 64) {
-                    hipLaunchKernelGGL((lu_trsm_kernel<T, kMaxPanel>), tgrid, dim3(kTrsmCols), 0, s, A, lda, n, k0, k1);
+                    hipLaunchKernelGGL((lu_trsm_kernel<T, kMaxPanel>), tgrid, tblock, 0, s, A, lda, n, k0, k1);
                     wide = true;
                 }
             }
-            if (!wide)
-                hipLaunchKernelGGL((lu_trsm_kernel<T, 64>), tgrid, dim3(kTrsmCols), 0, s, A, lda, n, k0, k1);
+            if (!wide) hipLaunchKernelGGL((lu_trsm_kernel<T, 64>), tgrid, tblock, 0, s, A, lda, n, k0, k1);
             if (use_mfma) {
                 const int rem = n - k1, bn = gemm_bn<T>();
                 hipLaunchKernelGGL(lu_gemm_mfma_kernel<T>, dim3((rem + kBM - 1) / kBM, (rem + bn - 1) / bn), dim3(256),

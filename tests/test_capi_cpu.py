@@ -52,11 +52,23 @@ def test_parse_run100_iter():
 
 
 def test_parse_extension_lines(tmp_path):
+    """"key value" lines after the mandatory ones select the extensions;
+    unknown keys and values are errors."""
     src = open(os.path.join(INPUTS, "run100_iter.in")).read()
     f = tmp_path / "run.in"
     f.write_text(src + "precision mixed\nwarm_start learned\nprofile 1\n")
     p = rf.Parameters.parse_parameter_file(str(f))
     assert p.precision == rf.PREC_MIXED and p.warm_start == rf.WARM_LEARNED and p.profile == 1
+    f.write_text(src + "precond auto\nprecond_switch_iters 45\nlu_factor_fp32 1\nrefine_tol 1d-13\n"
+                       "precond_neighbours 6\nmixed_inner_tol 1e-11\nlu_block 96\n")
+    p = rf.Parameters.parse_parameter_file(str(f))
+    assert (p.precond, p.precond_switch_iters, p.lu_factor_fp32, p.precond_neighbours, p.lu_block) == \
+        (rf.PRECOND_AUTO, 45, 1, 6, 96)
+    assert p.refine_tol == 1e-13 and p.mixed_inner_tol == 1e-11
+    for bad in ("precond sometimes\n", "warm_start later\n", "precision fp16\n"):
+        f.write_text(src + bad)
+        with pytest.raises(rf.RigidFibersError):
+            rf.Parameters.parse_parameter_file(str(f))
     f.write_text(src + "bogus 3\n")
     with pytest.raises(rf.RigidFibersError) as e:
         rf.Parameters.parse_parameter_file(str(f))

@@ -173,6 +173,11 @@ int rf_get_rhs(const rf_sim *sim, double *b);
 /* The assembled system matrix, promoted to FP64, into a caller buffer with
  * leading dimension lda >= n (test hook; the reference's AMat.out). */
 int rf_get_matrix(const rf_sim *sim, double *A, int64_t lda);
+/* Rows [row0, row0+rows) x columns [col0, col0+cols) of the same matrix into
+ * out (column-major, leading dimension ldo >= rows): spot checks of large
+ * systems without copying the whole matrix.  No counterpart in the reference. */
+int rf_get_matrix_block(const rf_sim *sim, int64_t row0, int64_t col0, int64_t rows, int64_t cols, double *out,
+                        int64_t ldo);
 int rf_get_stats(const rf_sim *sim, rf_stats *stats);
 /* Sets the coefficient vector (used as the next GMRES initial guess). */
 int rf_set_coefficients(rf_sim *sim, const double *x);

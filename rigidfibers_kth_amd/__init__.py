@@ -136,6 +136,8 @@ _SIGNATURES = {
     "rf_get_rhs": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "rf_get_matrix": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int64]),
     "rf_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(rf_stats)]),
+    "rf_get_matrix_block": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.c_int64, _dp, ctypes.c_int64]),
     "rf_set_coefficients": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "rf_matvec": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "rf_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
@@ -348,6 +350,13 @@ class Simulation:
         A = np.empty((self.n, self.n), order="F")
         _check(lib().rf_get_matrix(self._h, A.ctypes.data_as(_dp), self.n), "rf_get_matrix")
         return A
+
+    def matrix_block(self, row0, col0, rows, cols):
+        """A[row0:row0+rows, col0:col0+cols] promoted to FP64."""
+        out = np.empty((cols, rows))
+        _check(lib().rf_get_matrix_block(self._h, row0, col0, rows, cols, out.ctypes.data_as(_dp), rows),
+               "rf_get_matrix_block")
+        return out.T.copy()
 
     def matvec(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

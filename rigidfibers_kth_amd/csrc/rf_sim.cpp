@@ -1157,6 +1157,32 @@ extern "C" int rf_set_coefficients(rf_sim* s, const double* x)
     return RF_OK;
 }
 
+extern "C" int rf_get_matrix_block(const rf_sim* s, int64_t row0, int64_t col0, int64_t rows, int64_t cols,
+                                   double* out, int64_t ldo)
+{
+    CHECK_SIM(s);
+    if (!out || rows < 1 || cols < 1 || row0 < 0 || col0 < 0 || row0 + rows > s->n || col0 + cols > s->n ||
+        ldo < rows) {
+        set_error("rf_get_matrix_block: null buffer, block outside the n x n matrix, or ldo < rows");
+        return RF_EINVAL;
+    }
+    if (!s->mixed) {
+        const double* src = static_cast<const double*>(s->d_A) + col0 * s->lda + row0;
+        HIP_TRY(hipMemcpy2DAsync(out, sizeof(double) * ldo, src, sizeof(double) * s->lda, sizeof(double) * rows, cols,
+                                 hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        return RF_OK;
+    }
+    std::vector<float> tmp((size_t)rows * cols);
+    const float* src = static_cast<const float*>(s->d_A) + col0 * s->lda + row0;
+    HIP_TRY(hipMemcpy2DAsync(tmp.data(), sizeof(float) * rows, src, sizeof(float) * s->lda, sizeof(float) * rows, cols,
+                             hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int64_t c = 0; c < cols; ++c)
+        for (int64_t r = 0; r < rows; ++r) out[c * ldo + r] = tmp[(size_t)c * rows + r];
+    return RF_OK;
+}
+
 extern "C" int rf_get_matrix(const rf_sim* s, double* A, int64_t lda)
 {
     CHECK_SIM(s);
@@ -1164,19 +1190,7 @@ extern "C" int rf_get_matrix(const rf_sim* s, double* A, int64_t lda)
         set_error("rf_get_matrix: null buffer or lda < n");
         return RF_EINVAL;
     }
-    if (!s->mixed) {
-        HIP_TRY(hipMemcpy2DAsync(A, sizeof(double) * lda, s->d_A, sizeof(double) * s->lda, sizeof(double) * s->n,
-                                 s->n, hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        return RF_OK;
-    }
-    std::vector<float> tmp((size_t)s->n * s->n);
-    HIP_TRY(hipMemcpy2DAsync(tmp.data(), sizeof(float) * s->n, s->d_A, sizeof(float) * s->lda, sizeof(float) * s->n,
-                             s->n, hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    for (int c = 0; c < s->n; ++c)
-        for (int r = 0; r < s->n; ++r) A[(size_t)c * lda + r] = tmp[(size_t)c * s->n + r];
-    return RF_OK;
+    return rf_get_matrix_block(s, 0, 0, s->n, s->n, A, lda);
 }
 
 extern "C" int rf_get_stats(const rf_sim* s, rf_stats* st)

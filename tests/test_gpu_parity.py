@@ -279,3 +279,105 @@ def test_full_size_c2_properties():
     wm = np.concatenate([w[3 * t:3 * t + 3] for t in targets])
     assert np.abs(vm - vo).max() <= 2e-10 * np.abs(vo).max()
     assert np.abs(wm - wo).max() <= 2e-10 * np.abs(wo).max()
+
+
+def _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, tol, rng, pairs=16, rows=4):
+    """Sampled matrix blocks, identity diagonal blocks, rhs rows and velocity
+    rows against the oracle (relative to the largest entry of each check)."""
+    N = pos.size // 3
+    m = 3 * M
+    P3, T3 = pos.reshape(-1, 3), ori.reshape(-1, 3)
+    blocks = []
+    for _ in range(pairs):
+        i, j = rng.choice(N, 2, replace=False)
+        blk = oracle.matrix_block(1, EPS, P3[i], T3[i], P3[j], T3[j], M, noqi)
+        blocks.append((sim.matrix_block(i * m, j * m, m, m), blk))
+    scale = max(np.abs(o).max() for _, o in blocks)
+    for mine, blk in blocks:
+        assert np.abs(mine - blk).max() <= tol * scale
+    for i in rng.choice(N, 4, replace=False):
+        assert np.array_equal(sim.matrix_block(i * m, i * m, m, m), np.eye(m))
+    targets = rng.choice(N, rows, replace=False)
+    force = np.tile([0, 0, -1.0], N)
+    bo = oracle.assemble_rhs_rows(1, EPS, pos, ori, force, M, noqi, targets)
+    bm = np.concatenate([b[t * m:(t + 1) * m] for t in targets])
+    assert np.abs(bm - bo).max() <= tol * np.abs(bo).max()
+    vo, wo = oracle.compute_velocities_rows(1, EPS, pos, ori, force, M, noqi, x, targets)
+    vm = np.concatenate([v[3 * t:3 * t + 3] for t in targets])
+    wm = np.concatenate([w[3 * t:3 * t + 3] for t in targets])
+    assert np.abs(vm - vo).max() <= tol * np.abs(vo).max()
+    assert np.abs(wm - wo).max() <= tol * np.abs(wo).max()
+
+
+def test_full_size_n4000_properties():
+    """N=4000, volume fraction 5e-2 (n = 60000; 28.8 GB FP64 matrix) on the
+    mixed-precision path of the benchmark (FP64 operator, neighbour
+    preconditioner switching to FP32-LU GMRES-IR): sampled blocks, rhs and
+    velocity rows against the oracle, identity diagonal blocks, and the
+    backward error recomputed with the GPU matvec."""
+    N, M, noqi = 4000, 5, 8
+    pos, ori = _layout(N, 5e-2, 0x5EED0003)
+    p = _params(1, 0, gmres_restart=30, gmres_tol=1e-10, gmres_max_iters=3000, precond=rf.PRECOND_AUTO,
+                lu_factor_fp32=1, precond_switch_iters=10)
+    with rf.Simulation(rf.Configuration(p, pos, ori)) as sim:
+        sim.assemble_system()
+        b = sim.rhs()
+        sim.solve_system()
+        st = sim.stats()
+        x = sim.coefficients()
+        r = b - sim.matvec(x)
+        sim.update_velocities()
+        v, w = sim.velocities()
+        assert st.last_backward_error <= 1e-10
+        assert np.linalg.norm(r) <= 1.01e-10 * np.linalg.norm(b)
+        _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, 2e-10, np.random.default_rng(4))
+
+
+def test_full_size_n10000_mixed_properties():
+    """N=10000, volume fraction 1e-2 (n = 150000) with the FP32-stored
+    operator (90 GB matrix + 43 GB velocity operator on one GPU), neighbour
+    preconditioner, solve to 1e-8, then one full step at dt = 0.05: checks
+    against the FP64 oracle at the FP32 storage tolerance."""
+    N, M, noqi = 10000, 5, 8
+    pos, ori = _layout(N, 1e-2, 0x5EED0004)
+    p = _params(1, 0, gmres_restart=30, gmres_tol=1e-8, gmres_max_iters=1000, precision=rf.PREC_MIXED,
+                precond=rf.PRECOND_NEIGHBOUR, timestep=0.05)
+    with rf.Simulation(rf.Configuration(p, pos, ori)) as sim:
+        sim.assemble_system()
+        b = sim.rhs()
+        sim.solve_system()
+        st = sim.stats()
+        x = sim.coefficients()
+        r = b - sim.matvec(x)
+        sim.update_velocities()
+        v, w = sim.velocities()
+        assert st.last_solver == rf.SOLVER_GMRES and st.last_backward_error <= 1e-8
+        assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(b)
+        _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, 2e-6, np.random.default_rng(5), pairs=8)
+        X0, _ = sim.state()
+        sim.update_fibers()
+        X1, T1 = sim.state()
+    assert np.allclose(X1, X0 + 0.05 * v, rtol=0, atol=1e-12 * np.abs(X0).max())  # first step is Euler
+    assert np.allclose(np.linalg.norm(T1.reshape(-1, 3), axis=1), 1.0, atol=1e-14)
+
+
+def test_full_size_n2000_mixed_lu_properties():
+    """N=2000, volume fraction 2e-2 (n = 30000): FP32 matrix with FP64 near
+    field, FP32 LU on the matrix cores and iterative refinement against the
+    FP64-arithmetic residual of the stored operator to 1e-10."""
+    N, M, noqi = 2000, 5, 8
+    pos, ori = _layout(N, 2e-2, 0x5EED0002)
+    p = _params(1, 1, precision=rf.PREC_MIXED, mixed_inner_tol=1e-10)
+    with rf.Simulation(rf.Configuration(p, pos, ori)) as sim:
+        sim.assemble_system()
+        b = sim.rhs()
+        sim.solve_system()
+        st = sim.stats()
+        x = sim.coefficients()
+        r = b - sim.matvec(x)
+        sim.update_velocities()
+        v, w = sim.velocities()
+        assert st.last_solver == rf.SOLVER_LU and st.lu_factorizations == 1
+        assert st.last_backward_error <= 1e-10
+        assert np.linalg.norm(r) <= 1.01e-10 * np.linalg.norm(b)
+        _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, 2e-6, np.random.default_rng(6), pairs=12)

@@ -54,7 +54,7 @@ extern "C" {
 #define RF_PRECOND_NONE 0
 #define RF_PRECOND_NEIGHBOUR 1 /* block-Jacobi on each fiber + its K nearest fibers (restricted additive Schwarz) */
 #define RF_PRECOND_LU 2        /* LU factors of the stored matrix (FP32 in the mixed mode: GMRES-IR) */
-#define RF_PRECOND_AUTO 3      /* neighbour preconditioner; after precond_switch_iters iterations the LU */
+#define RF_PRECOND_AUTO 3      /* neighbour preconditioner; the LU once its convergence rate cannot meet the budget */
 
 /* output number formats */
 #define RF_FMT_F16_8 0  /* '(*(F16.8))' as the Fortran's *.out / tests/reference *.ref */

@@ -515,7 +515,7 @@ int rf_sim::factorize(bool* done)
         rf::set_error("LU: matrix is singular (U(" + std::to_string(info) + "," + std::to_string(info) + ") = 0)");
         return RF_ENOCONV;
     }
-    if (es == 4) {  // FP32 factors are applied every iteration: permutation + block inverses
+    if (es == 4) {  // FP32 factors are applied every iteration: permutation for the block sweeps
         if (!d_lu_aux) {
             void* q = nullptr;
             if (hipMalloc(&q, lu_solve_aux_bytes(n) + sizeof(double) * (size_t)lda) != hipSuccess) {
@@ -852,19 +852,32 @@ int rf_sim::solve()
         prec_kind = RF_PRECOND_NONE;
         const int maxit = p.gmres_max_iters;
         if (p.precond == RF_PRECOND_AUTO) {
-            // neighbour preconditioner within a budget, then GMRES on A (LU)^-1
-            // from the iterate reached
+            // Neighbour preconditioner within a budget (two restart cycles by
+            // default), then GMRES on A (LU)^-1 from the iterate reached.  After
+            // the first cycle the observed residual reduction per iteration
+            // predicts the iterations still needed; if they exceed the budget
+            // the switch happens at once.
             const int budget = std::min(maxit, p.precond_switch_iters > 0 ? p.precond_switch_iters : 2 * restart);
+            const double tol = p.gmres_tol;
             if ((rc = setup_precond(RF_PRECOND_NEIGHBOUR))) return rc;
-            if ((rc = solve_gmres(&iters, &be, &status, budget))) return rc;
+            if ((rc = solve_gmres(&iters, &be, &status, std::min(budget, restart)))) return rc;
+            const double r0 = stats.last_initial_residual;
+            if (status != 0 && iters < budget && r0 > 0.0 && be > 0.0 && be < r0) {
+                const double rate = std::pow(be / r0, 1.0 / iters);
+                const double needed = std::log(tol / be) / std::log(rate);
+                if (iters + needed <= budget) {
+                    int it2 = 0;
+                    if ((rc = solve_gmres(&it2, &be, &status, budget - iters))) return rc;
+                    iters += it2;
+                }
+            }
             if (status != 0 && iters < maxit) {
-                const double r0 = stats.last_initial_residual;
                 if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
                 int it2 = 0;
                 if ((rc = solve_gmres(&it2, &be, &status, maxit - iters))) return rc;
                 iters += it2;
-                stats.last_initial_residual = r0;
             }
+            stats.last_initial_residual = r0;
         } else {
             if (p.precond != RF_PRECOND_NONE && (rc = setup_precond(p.precond))) return rc;
             if ((rc = solve_gmres(&iters, &be, &status, maxit))) return rc;

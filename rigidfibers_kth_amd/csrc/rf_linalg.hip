@@ -105,7 +105,15 @@ __global__ void reduce_splits_kernel(const double* __restrict__ parts, int split
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int k = 0; k < splits; ++k) s += parts[(int64_t)k * stride + i];
+    int k = 0;
+    for (; k + 8 <= splits; k += 8) {  // eight loads in flight, summed in split order
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = parts[(int64_t)(k + u) * stride + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; k < splits; ++k) s += parts[(int64_t)k * stride + i];
     y[i] = (mode == 1) ? b[i] - s : s;
 }
 

@@ -713,7 +713,16 @@ __global__ void __launch_bounds__(256) sweep_block_kernel(const T* __restrict__ 
     const int i = LOWER ? k0 + bs + blockIdx.x * 256 + t : blockIdx.x * 256 + t;
     if (LOWER ? (i < n) : (i < k0)) {
         double acc = v[i];
-        for (int q = 0; q < bs; ++q) acc -= (double)LU[(int64_t)(k0 + q) * lda + i] * yk[q];
+        const T* li = LU + (int64_t)k0 * lda + i;
+        int q = 0;
+        for (; q + 8 <= bs; q += 8) {  // eight loads in flight
+            double a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = (double)li[(int64_t)(q + u) * lda];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc -= a[u] * yk[q + u];
+        }
+        for (; q < bs; ++q) acc -= (double)li[(int64_t)q * lda] * yk[q];
         v[i] = acc;
     }
 }

@@ -6,7 +6,8 @@ Headline workload (`value`): N=1000 rigid fibers of length 2, slenderness
 uniform on the sphere (seeded, overlaps rejected), gravity (0,0,-1); M=5
 Legendre terms, 8x3 Gauss points, analytic inner integrals, dt=0.1; solved the
 reference's way in FP64: GMRES(10) to a backward error of 1e-10, warm-started
-from the previous step, LU fallback (fortran/coeffs_and_vels.f90:253-339).
+from the previous step, LU fallback after 1000 iterations
+(fortran/coeffs_and_vels.f90:253-339).
 A "step" is one full timestep (assemble, solve, velocities, advance).
 
 Beside it, on the same initial state: the mixed-precision + learned path
@@ -14,7 +15,9 @@ Beside it, on the same initial state: the mixed-precision + learned path
 Arnoldi, neighbour preconditioner switching to FP32-LU GMRES-IR on hard
 systems, learned initial guess trained on the GPU on solved pairs), its
 relative position error against the plain run after the same steps, the FP64
-direct solver, and the same comparison at N=4000, volume fraction 5e-2.
+direct solver, and the same comparison at N=4000, volume fraction 5e-2, where
+the plain method is timed both as GMRES with LU fallback and as the FP64
+direct solver (the reference's two solver choices).
 
 Position differences are reported after one step and after all steps, both
 between the two methods and for each method against an FP64 run whose
@@ -50,7 +53,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemv_f64_n15000_traffic.json")
 C2 = dict(N=1000, phi=1e-2, seed=0x5EED0001)
 C4 = dict(N=4000, phi=5e-2, seed=0x5EED0003)
 # the reference's method: FP64 GMRES(10), previous-step guess, LU fallback
-PLAIN = dict(gmres_restart=10, gmres_max_iters=3000, gmres_tol=1e-10)
+# (fortran/coeffs_and_vels.f90:331-335) after 1000 iterations: beyond about
+# the cost of one FP64 LU in matvecs (~650 at N=1000, ~1500 at N=4000) more
+# GMRES iterations cannot pay off
+PLAIN = dict(gmres_restart=10, gmres_max_iters=1000, gmres_tol=1e-10)
 # mixed precision + learned: FP64 operator, residuals and Arnoldi; FP32 LU
 # factors on the matrix cores as the preconditioner of hard systems (GMRES-IR)
 # after two neighbour-preconditioned restart cycles; learned initial guess
@@ -196,14 +202,18 @@ def compare(rf, cfg, steps, warmup, train_states, label):
     log("%s: network trained on %d pairs" % (tag, tinfo["samples"]))
     plain, Xp = timed_run(rf, P0, T0, steps, warmup, **PLAIN)
     log("%s: plain FP64 %.3f steps/s" % (tag, plain["steps_per_sec"]))
+    direct, _ = timed_run(rf, P0, T0, steps, warmup, use_direct_solver=1)
+    log("%s: plain FP64 direct solver %.3f steps/s" % (tag, direct["steps_per_sec"]))
     impr, Xi = timed_run(rf, P0, T0, steps, warmup, weights=w, **IMPROVED)
     log("%s: mixed + learned %.3f steps/s" % (tag, impr["steps_per_sec"]))
     _, Xt = timed_run(rf, P0, T0, steps, warmup, **TIGHT)
     return {
         "workload": label,
         "plain_fp64": plain,
+        "plain_fp64_direct": direct,
         "mixed_learned": impr,
         "speedup": impr["steps_per_sec"] / plain["steps_per_sec"],
+        "speedup_vs_plain_direct": impr["steps_per_sec"] / direct["steps_per_sec"],
         "training": tinfo,
         **errors(Xi, Xp, Xt),
     }
@@ -307,7 +317,7 @@ def main():
         "data": "synthetic: seeded fiber layouts (N=1000 seed 0x5EED0001, N=4000 seed 0x5EED0003), overlaps rejected",
         "config": {
             "workload": "plain FP64, N=1000 fibers, volume fraction 1e-2, M=5, 8x3 Gauss points, analytic, "
-                        "GMRES(10) tol 1e-10 with LU fallback, previous-step warm start, dt=0.1",
+                        "GMRES(10) tol 1e-10 with LU fallback after 1000 iterations, previous-step warm start, dt=0.1",
             "fibers": C2["N"], "unknowns": n, "volume_fraction": C2["phi"], "solver": "gmres(10)", "tol": 1e-10,
         },
         "roofline": {

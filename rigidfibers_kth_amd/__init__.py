@@ -57,7 +57,8 @@ class rf_params(ctypes.Structure):
         ("precond_neighbours", ctypes.c_int32),
         ("precond_switch_iters", ctypes.c_int32),
         ("lu_factor_fp32", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("lu_column_panel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
         ("refine_tol", ctypes.c_double),
     ]
 
@@ -93,7 +94,8 @@ class rf_stats(ctypes.Structure):
         ("last_guess_beta", ctypes.c_double),
         ("last_precond", ctypes.c_int32),
         ("pad0", ctypes.c_int32),
-        ("reserved", ctypes.c_double * 4),
+        ("lu_panel_fallbacks", ctypes.c_int64),
+        ("reserved", ctypes.c_double * 3),
     ]
 
     def as_dict(self):
@@ -154,6 +156,8 @@ _SIGNATURES = {
                                     ctypes.c_float, ctypes.c_uint64, ctypes.c_int32, _fp]),
     "rf_write_step_outputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]),
     "rf_selftest_mfma_layout": (ctypes.c_int, [ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_float)]),
+    "rf_selftest_lu": (ctypes.c_int, [ctypes.c_int32] * 5 + [_dp, ctypes.POINTER(ctypes.c_int32),
+                                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "rf_num_fibers": (ctypes.c_int32, [ctypes.c_void_p]),
     "rf_num_unknowns": (ctypes.c_int64, [ctypes.c_void_p]),
     "rf_last_error": (ctypes.c_char_p, []),
@@ -197,6 +201,22 @@ def selftest_mfma_layout(mode):
     _check(lib().rf_selftest_mfma_layout(mode, _p(d), f.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
            "rf_selftest_mfma_layout")
     return d.reshape(64, 4), f.reshape(64, 4).astype(np.float64)
+
+
+def selftest_lu(A, elem_bytes=8, nb=0, column_panel=0, valu=0):
+    """The library's LU of an arbitrary square matrix (rf_selftest_lu):
+    returns (LU, ipiv, info, fallback) with LU the packed factors as LAPACK
+    getrf's, ipiv 0-based."""
+    a = np.array(A, dtype=np.float64, order="F", copy=True)
+    n = a.shape[0]
+    if a.shape != (n, n):
+        raise ValueError("square matrix expected")
+    ipiv = np.zeros(n, dtype=np.int32)
+    info, fb = ctypes.c_int32(0), ctypes.c_int32(0)
+    _check(lib().rf_selftest_lu(n, elem_bytes, nb, column_panel, valu, _p(a),
+                                ipiv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(info),
+                                ctypes.byref(fb)), "rf_selftest_lu")
+    return a, ipiv, int(info.value), int(fb.value)
 
 
 def _f(a):

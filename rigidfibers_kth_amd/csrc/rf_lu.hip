@@ -3,9 +3,9 @@
 //
 // Blocked right-looking factorisation, column-major, panel width NB:
 //   for each panel [k0, k1):
-//     for each column k of the panel: pivot search + row swap inside the
-//       panel + column scaling (one workgroup), rank-1 update of the rest of
-//       the panel (all CUs);
+//     either the block-pivoted panel (diagonal block in LDS, L21 = A21 U11^-1,
+//       checked against partial pivoting) or the column-by-column (Crout)
+//       panel with the pivot search over the whole column;
 //     apply the panel's row swaps to the other columns (laswp);
 //     U12 = L11^-1 A12 (unit lower triangular solve);
 //     A22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4f64 for FP64,
@@ -249,6 +249,168 @@ __global__ void __launch_bounds__(kTrsmC * kTrsmP) lu_trsm_kernel(T* A, int64_t 
         const int col = cbase + cc;
         if (col < n) A[(int64_t)col * lda + k0 + r] = S[r][cc];
     }
+}
+
+// ---- block-pivoted panel ---------------------------------------------------
+// Partial pivoting restricted to the panel's diagonal block: the block is
+// factorised by one workgroup in LDS, then L21 = A21 U11^-1 for all rows
+// below it.  If every multiplier of L21 satisfies |l| <= 1, each block pivot
+// was also a first largest entry of its whole column (rows below the block
+// come later in LAPACK idamax order), so pivots and factors are those of
+// partial pivoting, up to the order of the roundings.  Otherwise (or at a
+// zero pivot) info[1] is set: the factors are then not partial pivoting's
+// and the caller refactors the matrix with the column-by-column panel.
+// Matrices of the form I + (fiber coupling) never pivot, and this panel
+// costs two launches instead of one per column.
+
+// Diagonal block [k0, k0+nb) factorised with pivot search inside the block;
+// ipiv[k0..k0+nb) as LAPACK (0-based global rows).  One workgroup of 256.
+template <typename T, int NB>
+__global__ void __launch_bounds__(256) lu_diag_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv, int* info)
+{
+    __shared__ T S[NB][NB + 1];  // S[r][c] = A[k0 + r, k0 + c]
+    __shared__ int piv_s;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int e = t; e < nb * nb; e += 256) {
+        const int r = e % nb, c = e / nb;
+        S[r][c] = A[(int64_t)(k0 + c) * lda + k0 + r];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (wave == 0) {
+            double best = -1.0;
+            int bi = nb;
+            for (int r = j + lane; r < nb; r += 64) {
+                const double v = fabs((double)S[r][j]);
+                if (v > best) {
+                    best = v;
+                    bi = r;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (ov > best || (ov == best && oi < bi)) {
+                    best = ov;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                piv_s = bi;
+                ipiv[k0 + j] = k0 + bi;
+                if (!(best > 0.0)) info[1] = 1;  // zero pivot: left to the column-by-column panel
+            }
+        }
+        __syncthreads();
+        const int p = piv_s;
+        if (p != j)
+            for (int c = t; c < nb; c += 256) {
+                const T x = S[j][c];
+                S[j][c] = S[p][c];
+                S[p][c] = x;
+            }
+        __syncthreads();
+        const T piv = S[j][j];
+        const T inv = (piv != T(0)) ? T(1) / piv : T(0);
+        for (int r = j + 1 + t; r < nb; r += 256) S[r][j] *= inv;
+        __syncthreads();
+        for (int r = j + 1 + wave; r < nb; r += 4) {
+            const T l = S[r][j];
+            for (int c = j + 1 + lane; c < nb; c += 64) S[r][c] -= l * S[j][c];
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < nb * nb; e += 256) {
+        const int r = e % nb, c = e / nb;
+        A[(int64_t)(k0 + c) * lda + k0 + r] = S[r][c];
+    }
+}
+
+// Broadcast of lane SRC of each group of 4 lanes (DPP quad_perm).
+template <int SRC>
+__device__ __forceinline__ int quad_bcast_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, SRC * 0x55, 0xF, 0xF, false);
+}
+template <int SRC>
+__device__ __forceinline__ float quad_bcast(float v)
+{
+    return __int_as_float(quad_bcast_i<SRC>(__float_as_int(v)));
+}
+template <int SRC>
+__device__ __forceinline__ double quad_bcast(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)quad_bcast_i<SRC>((int)(unsigned)b);
+    const unsigned hi = (unsigned)quad_bcast_i<SRC>((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T>
+__device__ __forceinline__ T quad_bcast_dyn(T v, int src)
+{
+    switch (src) {
+    case 0: return quad_bcast<0>(v);
+    case 1: return quad_bcast<1>(v);
+    case 2: return quad_bcast<2>(v);
+    default: return quad_bcast<3>(v);
+    }
+}
+
+// L21 = A21 U11^-1 in place for rows [k0+nb, n) of the panel columns, and
+// the check |l| <= 1 (info[1] = 1 otherwise).  Each row is solved by 4
+// lanes of one quad, NB/4 columns each in registers; the solved entry x_c
+// is broadcast in the quad and every lane subtracts x_c U[c][d] from its
+// columns d.  LDS holds U11 strictly above the diagonal (zeros elsewhere,
+// so the update needs no masking) and 1/U[c][c]; 64 rows per workgroup.
+template <typename T, int NB>
+__global__ void __launch_bounds__(256) lu_l21_kernel(T* A, int64_t lda, int n, int k0, int nb, int* info)
+{
+    constexpr int CW = NB / 4;                  // columns per lane
+    constexpr int CP = CW + 16 / (int)sizeof(T); // group stride: 16-byte offsets spread the 4 groups over banks
+    __shared__ __attribute__((aligned(16))) T U[NB][4 * CP];
+    __shared__ T invd[NB];
+    const int k1 = k0 + nb;
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int c = e % NB, d = e / NB;  // U11[c][d]: consecutive threads read consecutive rows c
+        T v = T(0);
+        if (c < nb && d < nb && (d > c || d == c)) v = A[(int64_t)(k0 + d) * lda + k0 + c];
+        U[c][(d / CW) * CP + d % CW] = (d > c) ? v : T(0);
+        if (d == c) invd[c] = (c < nb) ? T(1) / v : T(0);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane & 3;
+    const int row = k1 + blockIdx.x * 64 + wave * 16 + (lane >> 2);
+    const bool live = row < n;
+    T a[CW];
+#pragma unroll
+    for (int k = 0; k < CW; ++k) {
+        const int col = g * CW + k;
+        a[k] = (live && col < nb) ? A[(int64_t)(k0 + col) * lda + row] : T(0);
+    }
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+#pragma unroll
+        for (int cl = 0; cl < CW; ++cl) {
+            const int c = gg * CW + cl;
+            const T xc = quad_bcast_dyn(a[cl] * invd[c], gg);  // x_c, solved by the quad's lane gg
+            if (g == gg) a[cl] = xc;
+            const T* u = &U[c][g * CP];
+#pragma unroll
+            for (int k = 0; k < CW; ++k) a[k] -= xc * u[k];
+        }
+    }
+    bool big = false;
+#pragma unroll
+    for (int k = 0; k < CW; ++k) {
+        const int col = g * CW + k;
+        if (live && col < nb) {
+            A[(int64_t)(k0 + col) * lda + row] = a[k];
+            big |= !(fabs((double)a[k]) <= 1.0);  // NaN included
+        }
+    }
+    if (big) atomicOr(&info[1], 1);
 }
 
 // ---- trailing update C -= L21 * U12 ------------------------------------
@@ -545,22 +707,42 @@ __global__ void trsv_update_kernel(const T* A, int64_t lda, double* x, int k0, i
     x[i] -= s;
 }
 
+// Block-pivoted panel [k0, k1): diagonal block, then L21 below it.
 template <typename T>
-hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, void* ws,
-                       hipStream_t s)
+void block_panel(T* A, int64_t lda, int n, int k0, int k1, int* ipiv, int* info, hipStream_t s)
+{
+    const int nb = k1 - k0;
+    const dim3 grid((n - k1 + 63) / 64);
+    if constexpr (sizeof(T) == 4) {
+        if (nb > 64) {
+            hipLaunchKernelGGL((lu_diag_kernel<T, kMaxPanel>), dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info);
+            if (k1 < n)
+                hipLaunchKernelGGL((lu_l21_kernel<T, kMaxPanel>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info);
+            return;
+        }
+    }
+    // FP64 panels are at most 64 wide
+    hipLaunchKernelGGL((lu_diag_kernel<T, 64>), dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info);
+    if (k1 < n) hipLaunchKernelGGL((lu_l21_kernel<T, 64>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info);
+}
+
+template <typename T>
+hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, int block_pivot,
+                       void* ws, hipStream_t s)
 {
     double* pval = static_cast<double*>(ws);
     int* pidx = reinterpret_cast<int*>(pval + kColMaxBlocks);
     unsigned int* counter = reinterpret_cast<unsigned int*>(pidx + kColMaxBlocks);
     for (int k0 = 0; k0 < n; k0 += nb) {
         const int k1 = std::min(n, k0 + nb);
-        for (int j = k0; j < k1; ++j) {
+        if (block_pivot) block_panel<T>(A, lda, n, k0, k1, ipiv, info, s);
+        for (int j = k0; j < k1 && !block_pivot; ++j) {
             const int rows = n - j;
             const int blocks = std::max(1, std::min(kColMaxBlocks, (rows + kColThreads - 1) / kColThreads));
             hipLaunchKernelGGL(lu_crout_kernel<T>, dim3(blocks), dim3(kColThreads), 0, s, A, lda, n, k0, k1, j, ipiv,
                                info, pval, pidx, counter);
         }
-        if (k1 < n)
+        if (k1 < n && !block_pivot)
             hipLaunchKernelGGL(lu_scale_kernel<T>, dim3((n - k1 + 255) / 256), dim3(256), 0, s, A, lda, n, k1 - 1);
         const int outside = n - (k1 - k0);
         if (outside > 0)
@@ -738,12 +920,12 @@ hipError_t mfma_probe(int mode, double* out64, float* out32, hipStream_t s)
 size_t lu_workspace_bytes() { return kColMaxBlocks * (sizeof(double) + sizeof(int)) + 64; }
 
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     void* ws, hipStream_t s)
+                     int block_pivot, void* ws, hipStream_t s)
 {
     if (nb < 1 || nb > (elem_bytes == 4 ? kMaxPanel : 64) || !ws || lda % 4 != 0) return hipErrorInvalidValue;
     if (elem_bytes == 8)
-        return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, ws, s);
-    return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, ws, s);
+        return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, ws, s);
+    return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, ws, s);
 }
 
 size_t lu_solve_aux_bytes(int n) { return ((sizeof(int) * (size_t)n + 255) / 256) * 256; }

@@ -475,7 +475,7 @@ int rf_sim::factorize(bool* done)
         }
         allocs.push_back(q);
         d_LU = q;
-        const size_t piv_bytes = (sizeof(int) * (n + 1) + 255) / 256 * 256;
+        const size_t piv_bytes = (sizeof(int) * (n + 2) + 255) / 256 * 256;
         if (hipMalloc(&q, piv_bytes + lu_workspace_bytes()) != hipSuccess) {
             rf::set_error("cannot allocate pivots");
             return RF_ENOMEM;
@@ -486,11 +486,6 @@ int rf_sim::factorize(bool* done)
         d_lu_ws = static_cast<char*>(q) + piv_bytes;
         HIP_TRY(hipMemsetAsync(d_lu_ws, 0, lu_workspace_bytes(), stream));
     }
-    if (es == (mixed ? 4 : 8))
-        HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
-    else  // FP32 factors of the FP64 matrix
-        HIP_TRY(launch_f64_to_f32(static_cast<const double*>(d_A), static_cast<float*>(d_LU), (size_t)lda * n, stream));
-    HIP_TRY(hipMemsetAsync(d_info, 0, sizeof(int), stream));
     // panel width: 128 for FP32 factors (halves the trailing update's traffic), 64 for FP64
     const int nb_max = (es == 4) ? 128 : 64;
     const int nb = (p.lu_block > 0) ? std::min(nb_max, p.lu_block) : nb_max;
@@ -500,19 +495,34 @@ int rf_sim::factorize(bool* done)
         if ((rc0 = timed_pair(&ea, &eb))) return rc0;
         HIP_TRY(hipEventRecord(ea, stream));
     }
-    HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, d_lu_ws, stream));
+    // the block-pivoted panel first; when its check finds that partial
+    // pivoting would have chosen another row, the copy is refactored with the
+    // column-by-column panel
+    int info[2] = {0, 0};
+    for (int block_pivot = p.lu_column_panel ? 0 : 1; block_pivot >= 0; --block_pivot) {
+        if (es == (mixed ? 4 : 8))
+            HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
+        else  // FP32 factors of the FP64 matrix
+            HIP_TRY(launch_f64_to_f32(static_cast<const double*>(d_A), static_cast<float*>(d_LU), (size_t)lda * n,
+                                      stream));
+        HIP_TRY(hipMemsetAsync(d_info, 0, 2 * sizeof(int), stream));
+        HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, block_pivot, d_lu_ws,
+                          stream));
+        HIP_TRY(hipMemcpyAsync(h_dots, d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::memcpy(info, h_dots, 2 * sizeof(int));
+        if (!block_pivot || !info[1]) break;
+        stats.lu_panel_fallbacks += 1;
+    }
     if (p.profile) {
         HIP_TRY(hipEventRecord(eb, stream));
         timed_push(ea, eb, kLU);
     }
     stats.lu_factorizations += 1;
-    int info = 0;
-    HIP_TRY(hipMemcpyAsync(h_dots, d_info, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     resolve_timed();
-    std::memcpy(&info, h_dots, sizeof(int));
-    if (info != 0) {
-        rf::set_error("LU: matrix is singular (U(" + std::to_string(info) + "," + std::to_string(info) + ") = 0)");
+    if (info[0] != 0) {
+        rf::set_error("LU: matrix is singular (U(" + std::to_string(info[0]) + "," + std::to_string(info[0]) + ") = 0)");
         return RF_ENOCONV;
     }
     if (es == 4) {  // FP32 factors are applied every iteration: permutation for the block sweeps
@@ -1474,6 +1484,58 @@ extern "C" int rf_selftest_mfma_layout(int32_t mode, double* out64, float* out32
         set_error(std::string("rf_selftest_mfma_layout: ") + hipGetErrorString(e));
         return RF_EHIP;
     }
+    return RF_OK;
+}
+
+extern "C" int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t column_panel, int32_t valu,
+                              double* a, int32_t* ipiv, int32_t* info, int32_t* fallback)
+{
+    const int nb_max = (elem_bytes == 4) ? 128 : 64;
+    if (n < 1 || n > 65536 || (elem_bytes != 4 && elem_bytes != 8) || nb < 0 || nb > nb_max || !a || !ipiv || !info ||
+        !fallback) {
+        set_error("rf_selftest_lu: bad arguments");
+        return RF_EINVAL;
+    }
+    if (nb == 0) nb = nb_max;
+    const int64_t lda = (n + 3) / 4 * 4;
+    const size_t es = (size_t)elem_bytes;
+    std::vector<char> host((size_t)lda * n * es, 0);
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t r = 0; r < n; ++r) {
+            const double v = a[c * n + r];
+            if (elem_bytes == 8)
+                reinterpret_cast<double*>(host.data())[c * lda + r] = v;
+            else
+                reinterpret_cast<float*>(host.data())[c * lda + r] = (float)v;
+        }
+    void *dA = nullptr, *dP = nullptr, *dW = nullptr;
+    hipError_t e = hipMalloc(&dA, host.size());
+    if (e == hipSuccess) e = hipMalloc(&dP, sizeof(int) * ((size_t)n + 2));
+    if (e == hipSuccess) e = hipMalloc(&dW, lu_workspace_bytes());
+    if (e == hipSuccess) e = hipMemcpy(dA, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dP, 0, sizeof(int) * ((size_t)n + 2));
+    if (e == hipSuccess) e = hipMemset(dW, 0, lu_workspace_bytes());
+    int* d_ipiv = static_cast<int*>(dP);
+    if (e == hipSuccess)
+        e = lu_factor(dA, elem_bytes, lda, n, nb, d_ipiv, d_ipiv + n, valu ? 0 : 1, column_panel ? 0 : 1, dW, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    std::vector<int> piv((size_t)n + 2);
+    if (e == hipSuccess) e = hipMemcpy(piv.data(), dP, sizeof(int) * ((size_t)n + 2), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(host.data(), dA, host.size(), hipMemcpyDeviceToHost);
+    if (dA) (void)hipFree(dA);
+    if (dP) (void)hipFree(dP);
+    if (dW) (void)hipFree(dW);
+    if (e != hipSuccess) {
+        set_error(std::string("rf_selftest_lu: ") + hipGetErrorString(e));
+        return RF_EHIP;
+    }
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t r = 0; r < n; ++r)
+            a[c * n + r] = (elem_bytes == 8) ? reinterpret_cast<double*>(host.data())[c * lda + r]
+                                             : (double)reinterpret_cast<float*>(host.data())[c * lda + r];
+    for (int i = 0; i < n; ++i) ipiv[i] = piv[i];
+    *info = piv[n];
+    *fallback = piv[n + 1];
     return RF_OK;
 }
 

@@ -139,13 +139,16 @@ def test_mfma_16x16x4_lane_layout():
     assert all(report.values()), (report, got[0][1][:20:4], got[1][1][:20:4])
 
 
-@pytest.mark.parametrize("valu,nb", [(1, 0), (0, 0), (0, 40), (0, 13)])
-def test_lu_solution_vs_oracle(valu, nb):
+@pytest.mark.parametrize("valu,nb,column_panel", [(1, 0, 0), (0, 0, 0), (0, 40, 0), (0, 13, 0), (0, 0, 1),
+                                                 (1, 13, 1)])
+def test_lu_solution_vs_oracle(valu, nb, column_panel):
     """Blocked LU (panel width nb, 0 = 64; partial last panels and trailing
-    tiles) on the vector ALUs or the matrix cores vs the oracle's dgesv."""
+    tiles) on the vector ALUs or the matrix cores, block-pivoted or
+    column-by-column panels, vs the oracle's dgesv."""
     N = 30
     pos, ori = _layout(N, 2e-2, 13)
-    with rf.Simulation(rf.Configuration(_params(1, 1, lu_valu_update=valu, lu_block=nb), pos, ori)) as sim:
+    p = _params(1, 1, lu_valu_update=valu, lu_block=nb, lu_column_panel=column_panel)
+    with rf.Simulation(rf.Configuration(p, pos, ori)) as sim:
         sim.assemble_system()
         sim.solve_system()
         x = sim.coefficients()
@@ -154,6 +157,7 @@ def test_lu_solution_vs_oracle(valu, nb):
     xo = oracle.lu_solve(A, b)
     assert np.abs(x - xo).max() <= 1e-12 * np.abs(xo).max()
     assert st.last_backward_error <= 1e-14
+    assert st.lu_panel_fallbacks == 0  # fiber systems never pivot
 
 
 def test_lu_and_gmres_agree_with_oracle():

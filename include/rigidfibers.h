@@ -93,12 +93,17 @@ typedef struct rf_params {
     int32_t lu_valu_update; /* 1: LU trailing update on the vector ALUs instead of the matrix cores */
     int32_t precond;        /* RF_PRECOND_*: right preconditioner of GMRES */
     int32_t precond_neighbours; /* K nearest fibers per preconditioner cluster (1..6, 0 = 4) */
-    int32_t precond_switch_iters; /* RF_PRECOND_AUTO: neighbour-preconditioned iterations before the LU (0 = 2 restarts) */
+    int32_t precond_switch_iters; /* RF_PRECOND_AUTO: neighbour-preconditioned iterations before the LU (0 = 2 restarts);
+                                     after a solve that ended on the LU the next start on it, every 8th
+                                     tries the neighbour preconditioner first again */
     int32_t lu_factor_fp32;       /* 1: FP32 LU factors also for the FP64 matrix (preconditioner / refinement) */
     int32_t lu_column_panel;      /* 1: LU pivot search over whole columns in every panel (0: inside the
-                                     panel's diagonal block, refactored this way when partial pivoting
-                                     would choose another row) */
-    int32_t reserved[2];
+                                     panel's diagonal block first; a panel where partial pivoting would
+                                     choose another row is redone over whole columns) */
+    int32_t lu_update_fp16;       /* 1: FP32 factors (mixed mode or lu_factor_fp32) that precondition GMRES
+                                     take trailing updates on fp16-rounded operands (FP32 accumulation);
+                                     the direct solver's factors keep FP32 updates */
+    int32_t reserved[1];
     double refine_tol;            /* GMRES tolerance once the LU preconditioner is in use (0 = gmres_tol) */
 } rf_params;
 
@@ -127,8 +132,8 @@ typedef struct rf_stats {
     double last_guess_alpha;        /* learned guess: x0 = alpha x_prev + beta MLP */
     double last_guess_beta;
     int32_t last_precond;           /* RF_PRECOND_* in use at the end of the last GMRES solve */
-    int32_t pad0;
-    int64_t lu_panel_fallbacks;     /* factorisations redone with the column-by-column panel */
+    int32_t last_neighbour_iters;   /* iterations of the last GMRES solve under the neighbour preconditioner */
+    int64_t lu_panel_fallbacks;     /* LU panels redone with the pivot search over whole columns */
     double reserved[3];
 } rf_stats;
 
@@ -142,7 +147,7 @@ void rf_params_default(rf_params *p);
  * Lines after the last mandatory one may hold "key value" extensions:
  * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1,
  * lu_block NB, precond none|neighbour|lu|auto, precond_neighbours K, precond_switch_iters I,
- * lu_factor_fp32 0|1, lu_column_panel 0|1, refine_tol T, mixed_inner_tol T. */
+ * lu_factor_fp32 0|1, lu_column_panel 0|1, lu_update_fp16 0|1, refine_tol T, mixed_inner_tol T. */
 int rf_parse_parameters(const char *path, rf_params *out);
 
 /* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per
@@ -253,12 +258,14 @@ int rf_selftest_copy(int64_t bytes, int32_t reps, double *ms);
  * below the diagonal).  elem_bytes 8: FP64; 4: a rounded to FP32 and
  * factorised in FP32.  nb: panel width (0 = 64 in FP64, 128 in FP32).
  * column_panel 1: pivot search over whole columns; 0: inside each panel's
- * diagonal block, and *fallback = 1 when partial pivoting would have chosen
- * another row (the factors are then invalid).  valu 1: trailing update on
- * the vector ALUs.  ipiv: n 0-based rows as LAPACK's (row k swapped with
+ * diagonal block first, and a panel where partial pivoting would have
+ * chosen another row is redone column by column (*fallback = number of such
+ * panels); the pivots are partial pivoting's either way.  valu 1: trailing update on
+ * the vector ALUs; update_fp16 1 (FP32 only): on the matrix cores with
+ * fp16-rounded operands (rf_params.lu_update_fp16).  ipiv: n 0-based rows as LAPACK's (row k swapped with
  * ipiv[k]); *info = k+1 for the first U(k,k) = 0. */
-int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t column_panel, int32_t valu, double *a,
-                   int32_t *ipiv, int32_t *info, int32_t *fallback);
+int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t column_panel, int32_t valu,
+                   int32_t update_fp16, double *a, int32_t *ipiv, int32_t *info, int32_t *fallback);
 
 /* Number of fibers / unknowns of a handle. */
 int32_t rf_num_fibers(const rf_sim *sim);

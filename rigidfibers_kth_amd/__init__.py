@@ -58,7 +58,8 @@ class rf_params(ctypes.Structure):
         ("precond_switch_iters", ctypes.c_int32),
         ("lu_factor_fp32", ctypes.c_int32),
         ("lu_column_panel", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("lu_update_fp16", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
         ("refine_tol", ctypes.c_double),
     ]
 
@@ -93,7 +94,7 @@ class rf_stats(ctypes.Structure):
         ("last_guess_alpha", ctypes.c_double),
         ("last_guess_beta", ctypes.c_double),
         ("last_precond", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("last_neighbour_iters", ctypes.c_int32),
         ("lu_panel_fallbacks", ctypes.c_int64),
         ("reserved", ctypes.c_double * 3),
     ]
@@ -156,7 +157,7 @@ _SIGNATURES = {
                                     ctypes.c_float, ctypes.c_uint64, ctypes.c_int32, _fp]),
     "rf_write_step_outputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]),
     "rf_selftest_mfma_layout": (ctypes.c_int, [ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_float)]),
-    "rf_selftest_lu": (ctypes.c_int, [ctypes.c_int32] * 5 + [_dp, ctypes.POINTER(ctypes.c_int32),
+    "rf_selftest_lu": (ctypes.c_int, [ctypes.c_int32] * 6 + [_dp, ctypes.POINTER(ctypes.c_int32),
                                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "rf_num_fibers": (ctypes.c_int32, [ctypes.c_void_p]),
     "rf_num_unknowns": (ctypes.c_int64, [ctypes.c_void_p]),
@@ -203,7 +204,7 @@ def selftest_mfma_layout(mode):
     return d.reshape(64, 4), f.reshape(64, 4).astype(np.float64)
 
 
-def selftest_lu(A, elem_bytes=8, nb=0, column_panel=0, valu=0):
+def selftest_lu(A, elem_bytes=8, nb=0, column_panel=0, valu=0, update_fp16=0):
     """The library's LU of an arbitrary square matrix (rf_selftest_lu):
     returns (LU, ipiv, info, fallback) with LU the packed factors as LAPACK
     getrf's, ipiv 0-based."""
@@ -213,7 +214,7 @@ def selftest_lu(A, elem_bytes=8, nb=0, column_panel=0, valu=0):
         raise ValueError("square matrix expected")
     ipiv = np.zeros(n, dtype=np.int32)
     info, fb = ctypes.c_int32(0), ctypes.c_int32(0)
-    _check(lib().rf_selftest_lu(n, elem_bytes, nb, column_panel, valu, _p(a),
+    _check(lib().rf_selftest_lu(n, elem_bytes, nb, column_panel, valu, update_fp16, _p(a),
                                 ipiv.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(info),
                                 ctypes.byref(fb)), "rf_selftest_lu")
     return a, ipiv, int(info.value), int(fb.value)

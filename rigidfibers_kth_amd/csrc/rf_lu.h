@@ -11,26 +11,30 @@ namespace rf {
 // float otherwise); ipiv[k] = row swapped with k (0-based); info[0] = k+1 if
 // U(k,k) == 0.  use_mfma selects the matrix-core trailing update; panel
 // width nb <= 128 for float, <= 64 for double.  ws:
-// lu_workspace_bytes() of device memory whose last word (the column
-// kernels' counter) is zero on entry and is left zero.
-// block_pivot = 1 searches pivots inside each panel's diagonal block only
-// and sets info[1] = 1 when that differs from partial pivoting (a multiplier
-// |l| > 1 or a zero pivot); the matrix is then overwritten with factors that
-// must be discarded, and the caller refactors the original with
-// block_pivot = 0 (pivot search over whole columns).  info[0..1] are zero on
-// entry.
-size_t lu_workspace_bytes();
+// lu_workspace_bytes(n) of device memory, zero on entry (the column
+// kernels' counter in it is left zero).
+// update_fp16 = 1 (float, use_mfma): trailing updates with fp16-rounded
+// operands and FP32 accumulation, for factors that precondition.
+// block_pivot = 1 searches each panel's pivots inside its diagonal block
+// first and checks the result against partial pivoting (every multiplier
+// |l| <= 1, no zero pivot); a panel that fails is restored and factorised
+// column by column (block_pivot = 0: every panel that way), so the pivots
+// are partial pivoting's either way.  *redone (may be null) receives the
+// number of panels redone.  Block pivoting synchronises the stream once per
+// panel.  info[0..1] are zero on entry.
+size_t lu_workspace_bytes(int n);
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     int block_pivot, void* ws, hipStream_t s);
+                     int block_pivot, int update_fp16, void* ws, int* redone, hipStream_t s);
 // x <- U^-1 L^-1 P x with FP64 vectors (substitution, two launches per
 // 64-row block).
 hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s);
 
 // The same solve with one launch per 64-row block and sweep, for the factors
-// that precondition every GMRES iteration.  aux holds lu_solve_aux_bytes(n):
-// the row permutation, filled by lu_prepare_solve once per factorisation;
-// tmp is n doubles.
-size_t lu_solve_aux_bytes(int n);
+// that precondition every GMRES iteration.  aux holds
+// lu_solve_aux_bytes(n, elem_bytes): the row permutation and the inverses of
+// the 64x64 diagonal blocks of L and U, filled by lu_prepare_solve once per
+// factorisation; tmp is n doubles.
+size_t lu_solve_aux_bytes(int n, int elem_bytes);
 hipError_t lu_prepare_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, void* aux,
                             hipStream_t s);
 hipError_t lu_solve_blocks(const void* LU, int elem_bytes, int64_t lda, int n, const void* aux, double* x,

@@ -9,7 +9,8 @@
 //     apply the panel's row swaps to the other columns (laswp);
 //     U12 = L11^-1 A12 (unit lower triangular solve);
 //     A22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4f64 for FP64,
-//       v_mfma_f32_16x16x4f32 for the FP32 factorisation of the mixed mode).
+//       v_mfma_f32_16x16x4f32 for FP32 factors, or v_mfma_f32_16x16x32_f16
+//       on fp16-rounded operands for FP32 factors that only precondition).
 // Pivots: first index of the largest |a_ik| (LAPACK idamax), so the pivot
 // sequence is dgesv's whenever the arithmetic agrees.
 #include "rf_internal.h"
@@ -17,6 +18,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <vector>
 
 namespace rf {
 namespace {
@@ -213,44 +215,6 @@ __global__ void lu_laswp_kernel(T* A, int64_t lda, int n, int k0, int k1, const 
     }
 }
 
-// U12 = L11^-1 A12: columns c in [k1, n), rows [k0, k1).  A workgroup of
-// 256 threads takes a 32-column slab into LDS (column-contiguous, coalesced
-// reads) with L11; the forward substitution runs row by row, the update of
-// each column's remaining rows split over 8 threads, one barrier per row.
-// LDS holds both in the factors' precision (FP32 factors use a 128-wide
-// panel: 64 KB + 17 KB).
-constexpr int kTrsmC = 32, kTrsmP = 8;
-
-template <typename T, int NB>
-__global__ void __launch_bounds__(kTrsmC * kTrsmP) lu_trsm_kernel(T* A, int64_t lda, int n, int k0, int k1)
-{
-    __shared__ T L[NB][NB];
-    __shared__ T S[NB][kTrsmC + 1];
-    const int nb = k1 - k0;
-    for (int e = threadIdx.x; e < nb * nb; e += kTrsmC * kTrsmP) {
-        const int r = e % nb, c = e / nb;
-        L[r][c] = A[(int64_t)(k0 + c) * lda + k0 + r];
-    }
-    const int cbase = k1 + blockIdx.x * kTrsmC;
-    for (int e = threadIdx.x; e < nb * kTrsmC; e += kTrsmC * kTrsmP) {
-        const int r = e % nb, c = e / nb;
-        const int col = cbase + c;
-        S[r][c] = (col < n) ? A[(int64_t)col * lda + k0 + r] : T(0);
-    }
-    __syncthreads();
-    const int c = threadIdx.x % kTrsmC, part = threadIdx.x / kTrsmC;
-    for (int r = 0; r < nb; ++r) {
-        const T xr = S[r][c];  // final: rows above r were applied in earlier steps
-        for (int q = r + 1 + part; q < nb; q += kTrsmP) S[q][c] -= L[q][r] * xr;
-        __syncthreads();
-    }
-    for (int e = threadIdx.x; e < nb * kTrsmC; e += kTrsmC * kTrsmP) {
-        const int r = e % nb, cc = e / nb;
-        const int col = cbase + cc;
-        if (col < n) A[(int64_t)col * lda + k0 + r] = S[r][cc];
-    }
-}
-
 // ---- block-pivoted panel ---------------------------------------------------
 // Partial pivoting restricted to the panel's diagonal block: the block is
 // factorised by one workgroup in LDS, then L21 = A21 U11^-1 for all rows
@@ -258,73 +222,160 @@ __global__ void __launch_bounds__(kTrsmC * kTrsmP) lu_trsm_kernel(T* A, int64_t 
 // was also a first largest entry of its whole column (rows below the block
 // come later in LAPACK idamax order), so pivots and factors are those of
 // partial pivoting, up to the order of the roundings.  Otherwise (or at a
-// zero pivot) info[1] is set: the factors are then not partial pivoting's
-// and the caller refactors the matrix with the column-by-column panel.
-// Matrices of the form I + (fiber coupling) never pivot, and this panel
-// costs two launches instead of one per column.
+// zero pivot) info[1] is set, the host restores the panel from the copy the
+// kernels took and factorises it column by column.  Most panels of fiber
+// systems pass (near contacts make a few fail), and a passing panel costs
+// two launches instead of one per column.
+
+// Reads the nb x nb block of A at (k0, k0) (nb <= NB) with 16 loads in
+// flight per thread and hands element (r, c) to f(r, c, v).  256 threads:
+// thread t covers row t % NB of the columns t / NB + (256 / NB) u, so each
+// load instruction reads consecutive rows of one column.
+template <typename T, int NB, typename F>
+__device__ __forceinline__ void load_block(const T* A, int64_t lda, int k0, int nb, F&& f)
+{
+    constexpr int H = 256 / NB, PER = NB / H, BATCH = 16;
+    static_assert(PER % BATCH == 0, "columns per thread");
+    const int r = threadIdx.x % NB, c0 = threadIdx.x / NB;
+#pragma unroll
+    for (int u0 = 0; u0 < PER; u0 += BATCH) {
+        T v[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            const int c = c0 + H * (u0 + u);
+            v[u] = (r < nb && c < nb) ? A[(int64_t)(k0 + c) * lda + k0 + r] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            const int c = c0 + H * (u0 + u);
+            if (r < nb && c < nb) f(r, c, v[u]);
+        }
+    }
+}
 
 // Diagonal block [k0, k0+nb) factorised with pivot search inside the block;
-// ipiv[k0..k0+nb) as LAPACK (0-based global rows).  One workgroup of 256.
+// ipiv[k0..k0+nb) as LAPACK (0-based global rows); bak receives the block's
+// original values.  One workgroup of 256: thread t owns column t % NB and the
+// rows r = t / NB (mod 256 / NB).  Two barriers per column: (A) the pivot
+// row swap and the scaling of column j, (B) the rank-1 update, in which the
+// owners of column j+1 also find their partial argmax for the next column.
+// The update loads 8 rows before it stores any (LDS latency overlapped).
 template <typename T, int NB>
-__global__ void __launch_bounds__(256) lu_diag_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv, int* info)
+__global__ void __launch_bounds__(256) lu_diag_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv, int* info,
+                                                      T* __restrict__ bak, int64_t ldb)
 {
+    constexpr int H = 256 / NB;  // row phases
     __shared__ T S[NB][NB + 1];  // S[r][c] = A[k0 + r, k0 + c]
-    __shared__ int piv_s;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int e = t; e < nb * nb; e += 256) {
-        const int r = e % nb, c = e / nb;
-        S[r][c] = A[(int64_t)(k0 + c) * lda + k0 + r];
+    __shared__ double cand_v[H];
+    __shared__ int cand_i[H];
+    __shared__ T cand_s[H];
+    __shared__ T djj_s;
+    __shared__ int piv_l[NB];  // pivots and the zero-pivot flag stay in LDS until the end: a global store
+    __shared__ int zero_l;     // before a barrier would make every barrier wait for it
+    const int t = threadIdx.x, c = t % NB, h = t / NB;
+    if (t == 0) zero_l = 0;
+    load_block<T, NB>(A, lda, k0, nb, [&](int r, int cc, T v) {
+        S[r][cc] = v;
+        bak[(int64_t)cc * ldb + r] = v;
+    });
+    __syncthreads();
+    if (c == 0) {  // partial argmax of column 0
+        double best = -1.0;
+        int bi = nb;
+        T bs = T(0);
+        for (int r = h; r < nb; r += H) {
+            const double v = fabs((double)S[r][0]);
+            if (v > best) {
+                best = v;
+                bi = r;
+                bs = S[r][0];
+            }
+        }
+        cand_v[h] = best;
+        cand_i[h] = bi;
+        cand_s[h] = bs;
+        if (h == 0) djj_s = S[0][0];
     }
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
-        if (wave == 0) {
-            double best = -1.0;
-            int bi = nb;
-            for (int r = j + lane; r < nb; r += 64) {
-                const double v = fabs((double)S[r][j]);
-                if (v > best) {
-                    best = v;
-                    bi = r;
-                }
-            }
+        // (A) pivot of column j: first largest |a_rj|, r >= j
+        double best = cand_v[0];
+        int p = cand_i[0];
+        T pv = cand_s[0];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ov > best || (ov == best && oi < bi)) {
-                    best = ov;
-                    bi = oi;
-                }
+        for (int q = 1; q < H; ++q)
+            if (cand_v[q] > best || (cand_v[q] == best && cand_i[q] < p)) {
+                best = cand_v[q];
+                p = cand_i[q];
+                pv = cand_s[q];
             }
-            if (lane == 0) {
-                piv_s = bi;
-                ipiv[k0 + j] = k0 + bi;
-                if (!(best > 0.0)) info[1] = 1;  // zero pivot: left to the column-by-column panel
-            }
+        const T djj = djj_s;
+        if (t == 0) {
+            piv_l[j] = p;
+            if (!(best > 0.0)) zero_l = 1;  // zero pivot: left to the column-by-column panel
+            S[j][j] = pv;
+        }
+        const T inv = (pv != T(0)) ? T(1) / pv : T(0);
+        if (p != j && h == 0 && c != j && c < nb) {
+            const T x = S[j][c];
+            S[j][c] = S[p][c];
+            S[p][c] = x;
+        }
+        {
+            const int r = j + 1 + t;
+            if (r < nb) S[r][j] = ((r == p) ? djj : S[r][j]) * inv;
         }
         __syncthreads();
-        const int p = piv_s;
-        if (p != j)
-            for (int c = t; c < nb; c += 256) {
-                const T x = S[j][c];
-                S[j][c] = S[p][c];
-                S[p][c] = x;
+        // (B) trailing update; column j+1's owners keep the partial argmax
+        if (c > j && c < nb) {
+            const T pj = S[j][c];
+            const bool next = (c == j + 1);
+            double nbest = -1.0;
+            int nbi = nb;
+            T nbs = T(0);
+            const int r0 = j + 1 + ((h - (j + 1)) % H + H) % H;
+            for (int rb = r0; rb < nb; rb += 8 * H) {
+                T l[8], v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int r = rb + u * H;
+                    if (r < nb) {
+                        l[u] = S[r][j];
+                        v[u] = S[r][c];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int r = rb + u * H;
+                    if (r < nb) {
+                        v[u] -= l[u] * pj;
+                        S[r][c] = v[u];
+                        if (next) {
+                            const double a = fabs((double)v[u]);
+                            if (a > nbest) {
+                                nbest = a;
+                                nbi = r;
+                                nbs = v[u];
+                            }
+                            if (r == j + 1) djj_s = v[u];
+                        }
+                    }
+                }
             }
-        __syncthreads();
-        const T piv = S[j][j];
-        const T inv = (piv != T(0)) ? T(1) / piv : T(0);
-        for (int r = j + 1 + t; r < nb; r += 256) S[r][j] *= inv;
-        __syncthreads();
-        for (int r = j + 1 + wave; r < nb; r += 4) {
-            const T l = S[r][j];
-            for (int c = j + 1 + lane; c < nb; c += 64) S[r][c] -= l * S[j][c];
+            if (next) {
+                cand_v[h] = nbest;
+                cand_i[h] = nbi;
+                cand_s[h] = nbs;
+            }
         }
         __syncthreads();
     }
     for (int e = t; e < nb * nb; e += 256) {
-        const int r = e % nb, c = e / nb;
-        A[(int64_t)(k0 + c) * lda + k0 + r] = S[r][c];
+        const int r = e % nb, cc = e / nb;
+        A[(int64_t)(k0 + cc) * lda + k0 + r] = S[r][cc];
     }
+    if (t < nb) ipiv[k0 + t] = k0 + piv_l[t];
+    if (t == 0 && zero_l) info[1] = 1;
 }
 
 // Broadcast of lane SRC of each group of 4 lanes (DPP quad_perm).
@@ -364,20 +415,21 @@ __device__ __forceinline__ T quad_bcast_dyn(T v, int src)
 // columns d.  LDS holds U11 strictly above the diagonal (zeros elsewhere,
 // so the update needs no masking) and 1/U[c][c]; 64 rows per workgroup.
 template <typename T, int NB>
-__global__ void __launch_bounds__(256) lu_l21_kernel(T* A, int64_t lda, int n, int k0, int nb, int* info)
+__global__ void __launch_bounds__(256) lu_l21_kernel(T* A, int64_t lda, int n, int k0, int nb, int* info,
+                                                     T* __restrict__ bak, int64_t ldb)
 {
     constexpr int CW = NB / 4;                  // columns per lane
     constexpr int CP = CW + 16 / (int)sizeof(T); // group stride: 16-byte offsets spread the 4 groups over banks
     __shared__ __attribute__((aligned(16))) T U[NB][4 * CP];
     __shared__ T invd[NB];
     const int k1 = k0 + nb;
-    for (int e = threadIdx.x; e < NB * NB; e += 256) {
-        const int c = e % NB, d = e / NB;  // U11[c][d]: consecutive threads read consecutive rows c
-        T v = T(0);
-        if (c < nb && d < nb && (d > c || d == c)) v = A[(int64_t)(k0 + d) * lda + k0 + c];
-        U[c][(d / CW) * CP + d % CW] = (d > c) ? v : T(0);
-        if (d == c) invd[c] = (c < nb) ? T(1) / v : T(0);
-    }
+    for (int e = threadIdx.x; e < NB * 4 * CP; e += 256) (&U[0][0])[e] = T(0);
+    for (int e = threadIdx.x; e < NB; e += 256) invd[e] = T(0);
+    __syncthreads();
+    load_block<T, NB>(A, lda, k0, nb, [&](int c, int d, T v) {  // U11[c][d]
+        if (d > c) U[c][(d / CW) * CP + d % CW] = v;
+        if (d == c) invd[c] = T(1) / v;
+    });
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane & 3;
@@ -388,6 +440,7 @@ __global__ void __launch_bounds__(256) lu_l21_kernel(T* A, int64_t lda, int n, i
     for (int k = 0; k < CW; ++k) {
         const int col = g * CW + k;
         a[k] = (live && col < nb) ? A[(int64_t)(k0 + col) * lda + row] : T(0);
+        if (live && col < nb) bak[(int64_t)col * ldb + (row - k0)] = a[k];
     }
 #pragma unroll
     for (int gg = 0; gg < 4; ++gg) {
@@ -411,6 +464,97 @@ __global__ void __launch_bounds__(256) lu_l21_kernel(T* A, int64_t lda, int n, i
         }
     }
     if (big) atomicOr(&info[1], 1);
+}
+
+// U12 = L11^-1 A12 (L11 unit lower) in place for the columns [k0+nb, n) of
+// the panel rows, and the per-column powers of two for the fp16 update
+// (colscale may be null): the largest |u| of each column is taken into
+// [2^11, 2^12).  Each column is solved by the 4 lanes of a quad, NB/4 rows
+// each in registers (the mirror of lu_l21_kernel): x_r is final once the
+// rows above it are applied; it is broadcast in the quad and every lane
+// subtracts L[q][r] x_r from its rows q > r.  64 columns per workgroup.
+template <typename T, int NB>
+__global__ void __launch_bounds__(256) lu_u12_kernel(T* A, int64_t lda, int n, int k0, int nb,
+                                                     float* __restrict__ colscale)
+{
+    constexpr int CW = NB / 4;
+    constexpr int CP = CW + 16 / (int)sizeof(T);
+    constexpr int VL = 16 / (int)sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VL)));
+    __shared__ __attribute__((aligned(16))) T Lt[NB][4 * CP];  // Lt[r][g*CP + k] = L11[g*CW + k][r] below the diagonal
+    const int k1 = k0 + nb;
+    for (int e = threadIdx.x; e < NB * 4 * CP; e += 256) (&Lt[0][0])[e] = T(0);
+    __syncthreads();
+    load_block<T, NB>(A, lda, k0, nb, [&](int q, int r, T v) {  // L11[q][r]
+        if (q > r) Lt[r][(q / CW) * CP + q % CW] = v;
+    });
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane & 3;
+    const int col = k1 + blockIdx.x * 64 + wave * 16 + (lane >> 2);
+    const bool live = col < n;
+    T* src = A + (int64_t)col * lda + k0 + g * CW;
+    const bool vec = live && (k0 % VL == 0) && (lda % VL == 0) && (g * CW + CW <= nb);
+    T a[CW];
+    if (vec) {
+#pragma unroll
+        for (int k = 0; k < CW; k += VL) {
+            const vec_t v = *reinterpret_cast<const vec_t*>(src + k);
+#pragma unroll
+            for (int u = 0; u < VL; ++u) a[k + u] = v[u];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CW; ++k) a[k] = (live && g * CW + k < nb) ? src[k] : T(0);
+    }
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+#pragma unroll
+        for (int cl = 0; cl < CW; ++cl) {
+            const T xr = quad_bcast_dyn(a[cl], gg);  // x_r from the quad's lane gg
+            const T* l = &Lt[gg * CW + cl][g * CP];
+#pragma unroll
+            for (int k = 0; k < CW; ++k) a[k] -= xr * l[k];
+        }
+    }
+    float m = 0.f;
+    if (vec) {
+#pragma unroll
+        for (int k = 0; k < CW; k += VL) {
+            vec_t v;
+#pragma unroll
+            for (int u = 0; u < VL; ++u) {
+                v[u] = a[k + u];
+                m = fmaxf(m, fabsf((float)a[k + u]));
+            }
+            *reinterpret_cast<vec_t*>(src + k) = v;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CW; ++k)
+            if (live && g * CW + k < nb) {
+                src[k] = a[k];
+                m = fmaxf(m, fabsf((float)a[k]));
+            }
+    }
+    if (colscale) {
+        m = fmaxf(m, __shfl_xor(m, 1, 64));
+        m = fmaxf(m, __shfl_xor(m, 2, 64));
+        if (g == 0 && live) {
+            const int e = (m > 0.f && m <= 3.0e38f) ? ilogbf(m) : 11;
+            colscale[col] = ldexpf(1.f, min(60, max(-60, 11 - e)));
+        }
+    }
+}
+
+// Panel columns [k0, k0+nb), rows [k0, n), back from the copy the two
+// kernels above took (a panel whose check failed).
+template <typename T>
+__global__ void lu_restore_kernel(T* A, int64_t lda, int n, int k0, int nb, const T* __restrict__ bak, int64_t ldb)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (k0 + r < n && c < nb) A[(int64_t)(k0 + c) * lda + k0 + r] = bak[(int64_t)c * ldb + r];
 }
 
 // ---- trailing update C -= L21 * U12 ------------------------------------
@@ -653,6 +797,157 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, in
         }
 }
 
+// fp16 matrix-core update of FP32 factors (preconditioner factors only):
+// C[:, j] = (C[:, j] s_j + (-L21) (U12[:, j] s_j)) / s_j with the operands
+// rounded to fp16 and FP32 accumulation (v_mfma_f32_16x16x32_f16).  |L21| <= 1
+// under partial pivoting, and s_j (a power of two from the U12 solve) puts
+// column j of U12 into [2^11, 2^12), inside fp16's normal range.  Tile 128 x
+// 128, 4 wavefronts of 64 x 64 (16x16 blocks), K staged 32 at a time in LDS
+// as fp16 (double-buffered, global loads one chunk ahead); the accumulators
+// start as the scaled C tile.  The operand layout follows the 16x16x4 f32
+// instruction: lane l supplies row l%16 of A and column l%16 of B at the
+// K slots of its group l/16 (8 consecutive K here), and holds
+// D[4(l/16)+r][l%16].
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+constexpr int kHB = 128, kHK = 32, kHPad = 8;
+
+struct HalfStage {
+    f4 a[4];  // A: rows 4*(t%32) .. +3 at K columns 4*(t/32) .. +3
+    f4 b[2][2];  // B: column v/4, K 8*(v%4) .. +7 for v = t, t + 256
+    float bs[2];
+
+    __device__ __forceinline__ void load(const float* A, int64_t lda, int n, int rbase, int cbase, int kc, int kcount,
+                                         bool vec, const float* __restrict__ colscale)
+    {
+        const int t = threadIdx.x;
+        const int r0 = rbase + 4 * (t % 32), kk0 = 4 * (t / 32);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int kk = kk0 + q;
+            const float* src = A + (int64_t)(kc + kk) * lda + r0;
+            if (vec && kk < kcount && r0 + 3 < n) {
+                a[q] = *reinterpret_cast<const f4*>(src);
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[q][u] = (kk < kcount && r0 + u < n) ? src[u] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int v = t + 256 * h;
+            const int col = cbase + v / 4, kb = 8 * (v % 4);
+            const float* src = A + (int64_t)col * lda + kc + kb;
+            bs[h] = (col < n) ? colscale[col] : 0.f;
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                if (vec && col < n && kb + 4 * g + 3 < kcount) {
+                    b[h][g] = *reinterpret_cast<const f4*>(src + 4 * g);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        b[h][g][u] = (col < n && kb + 4 * g + u < kcount) ? src[4 * g + u] : 0.f;
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void store(_Float16 (&As)[kHB][kHK + kHPad], _Float16 (&Bs)[kHB][kHK + kHPad]) const
+    {
+        const int t = threadIdx.x;
+        const int r0 = 4 * (t % 32), kk0 = 4 * (t / 32);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            h4 v;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = (_Float16)(-a[q][u]);
+            *reinterpret_cast<h4*>(&As[r0 + u][kk0]) = v;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int v = t + 256 * h;
+            h8 w;
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) w[4 * g + u] = (_Float16)(b[h][g][u] * bs[h]);
+            *reinterpret_cast<h8*>(&Bs[v / 4][8 * (v % 4)]) = w;
+        }
+    }
+};
+
+__global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda, int n, int k0, int k1,
+                                                         const float* __restrict__ colscale)
+{
+    __shared__ __attribute__((aligned(16))) _Float16 As[2][kHB][kHK + kHPad];
+    __shared__ __attribute__((aligned(16))) _Float16 Bs[2][kHB][kHK + kHPad];
+    const int rbase = k1 + blockIdx.x * kHB, cbase = k1 + blockIdx.y * kHB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    const int lr = lane & 15, lg = lane >> 4;
+    const bool vec = (k0 % 4 == 0) && (k1 % 4 == 0) && (lda % 4 == 0);
+    float sc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = cbase + wc + 16 * j + lr;
+        sc[j] = (col < n) ? colscale[col] : 1.f;
+    }
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row0 = rbase + wr + 16 * i + 4 * lg;
+            const int col = cbase + wc + 16 * j + lr;
+            const float* p = A + (int64_t)col * lda + row0;
+            if (vec && col < n && row0 + 3 < n) {
+                acc[i][j] = *reinterpret_cast<const f4*>(p);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[i][j][r] = (col < n && row0 + r < n) ? p[r] : 0.f;
+            }
+        }
+    HalfStage st;
+    st.load(A, lda, n, rbase, cbase, k0, min(kHK, k1 - k0), vec, colscale);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] *= sc[j];
+    int buf = 0;
+    st.store(As[0], Bs[0]);
+    for (int kc = k0; kc < k1; kc += kHK) {
+        __syncthreads();  // chunk kc is in As/Bs[buf]; the other buffer's readers are done
+        if (kc + kHK < k1) st.load(A, lda, n, rbase, cbase, kc + kHK, min(kHK, k1 - kc - kHK), vec, colscale);
+        h8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const h8*>(&As[buf][wr + 16 * i + lr][8 * lg]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const h8*>(&Bs[buf][wc + 16 * j + lr][8 * lg]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (kc + kHK < k1) st.store(As[buf ^ 1], Bs[buf ^ 1]);
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row0 = rbase + wr + 16 * i + 4 * lg;
+            const int col = cbase + wc + 16 * j + lr;
+            if (col >= n) continue;
+            const f4 c = acc[i][j] * (1.f / sc[j]);
+            float* p = A + (int64_t)col * lda + row0;
+            if (vec && row0 + 3 < n) {
+                *reinterpret_cast<f4*>(p) = c;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (row0 + r < n) p[r] = c[r];
+            }
+        }
+}
+
 // ---- solves with the factors (FP64 vectors) ------------------------------
 
 __global__ void apply_pivots_kernel(double* b, int n, const int* __restrict__ ipiv)
@@ -707,58 +1002,93 @@ __global__ void trsv_update_kernel(const T* A, int64_t lda, double* x, int k0, i
     x[i] -= s;
 }
 
-// Block-pivoted panel [k0, k1): diagonal block, then L21 below it.
+// Block-pivoted panel [k0, k1): diagonal block, then L21 below it; bak
+// receives the panel's original columns (rows [k0, n), leading dimension n).
 template <typename T>
-void block_panel(T* A, int64_t lda, int n, int k0, int k1, int* ipiv, int* info, hipStream_t s)
+void block_panel(T* A, int64_t lda, int n, int k0, int k1, int* ipiv, int* info, T* bak, hipStream_t s)
 {
     const int nb = k1 - k0;
     const dim3 grid((n - k1 + 63) / 64);
     if constexpr (sizeof(T) == 4) {
         if (nb > 64) {
-            hipLaunchKernelGGL((lu_diag_kernel<T, kMaxPanel>), dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info);
+            hipLaunchKernelGGL((lu_diag_kernel<T, kMaxPanel>), dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info,
+                               bak, (int64_t)n);
             if (k1 < n)
-                hipLaunchKernelGGL((lu_l21_kernel<T, kMaxPanel>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info);
+                hipLaunchKernelGGL((lu_l21_kernel<T, kMaxPanel>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info, bak,
+                                   (int64_t)n);
             return;
         }
     }
     // FP64 panels are at most 64 wide
-    hipLaunchKernelGGL((lu_diag_kernel<T, 64>), dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info);
-    if (k1 < n) hipLaunchKernelGGL((lu_l21_kernel<T, 64>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info);
+    hipLaunchKernelGGL((lu_diag_kernel<T, 64>), dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info, bak, (int64_t)n);
+    if (k1 < n)
+        hipLaunchKernelGGL((lu_l21_kernel<T, 64>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info, bak, (int64_t)n);
 }
+
+constexpr size_t kWsScaleOffset = kColMaxBlocks * (sizeof(double) + sizeof(int)) + 64;
+size_t ws_backup_offset(int n) { return kWsScaleOffset + (sizeof(float) * (size_t)n + 255) / 256 * 256; }
 
 template <typename T>
 hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, int block_pivot,
-                       void* ws, hipStream_t s)
+                       int update_fp16, void* ws, int* redone, hipStream_t s)
 {
     double* pval = static_cast<double*>(ws);
     int* pidx = reinterpret_cast<int*>(pval + kColMaxBlocks);
     unsigned int* counter = reinterpret_cast<unsigned int*>(pidx + kColMaxBlocks);
+    const bool f16 = (sizeof(T) == 4) && use_mfma && update_fp16;
+    float* colscale = f16 ? reinterpret_cast<float*>(static_cast<char*>(ws) + kWsScaleOffset) : nullptr;
+    T* bak = reinterpret_cast<T*>(static_cast<char*>(ws) + ws_backup_offset(n));
+    if (redone) *redone = 0;
     for (int k0 = 0; k0 < n; k0 += nb) {
         const int k1 = std::min(n, k0 + nb);
-        if (block_pivot) block_panel<T>(A, lda, n, k0, k1, ipiv, info, s);
-        for (int j = k0; j < k1 && !block_pivot; ++j) {
+        bool columns = !block_pivot;
+        if (block_pivot) {
+            block_panel<T>(A, lda, n, k0, k1, ipiv, info, bak, s);
+            // the check decides before anything else reads the panel
+            int flag = 0;
+            hipError_t e = hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+            if (flag) {
+                hipLaunchKernelGGL(lu_restore_kernel<T>, dim3((n - k0 + 255) / 256, k1 - k0), dim3(256), 0, s, A, lda,
+                                   n, k0, k1 - k0, bak, (int64_t)n);
+                e = hipMemsetAsync(info + 1, 0, sizeof(int), s);
+                if (e != hipSuccess) return e;
+                columns = true;
+                if (redone) *redone += 1;
+            }
+        }
+        for (int j = k0; j < k1 && columns; ++j) {
             const int rows = n - j;
             const int blocks = std::max(1, std::min(kColMaxBlocks, (rows + kColThreads - 1) / kColThreads));
             hipLaunchKernelGGL(lu_crout_kernel<T>, dim3(blocks), dim3(kColThreads), 0, s, A, lda, n, k0, k1, j, ipiv,
                                info, pval, pidx, counter);
         }
-        if (k1 < n && !block_pivot)
+        if (k1 < n && columns)
             hipLaunchKernelGGL(lu_scale_kernel<T>, dim3((n - k1 + 255) / 256), dim3(256), 0, s, A, lda, n, k1 - 1);
         const int outside = n - (k1 - k0);
         if (outside > 0)
             hipLaunchKernelGGL(lu_laswp_kernel<T>, dim3((outside + 255) / 256), dim3(256), 0, s, A, lda, n, k0, k1,
                                ipiv);
         if (k1 < n) {
-            const dim3 tgrid((n - k1 + kTrsmC - 1) / kTrsmC), tblock(kTrsmC * kTrsmP);
+            const dim3 ugrid((n - k1 + 63) / 64);
             bool wide = false;
             if constexpr (sizeof(T) == 4) {
                 if (nb > 64) {
-                    hipLaunchKernelGGL((lu_trsm_kernel<T, kMaxPanel>), tgrid, tblock, 0, s, A, lda, n, k0, k1);
+                    hipLaunchKernelGGL((lu_u12_kernel<T, kMaxPanel>), ugrid, dim3(256), 0, s, A, lda, n, k0, k1 - k0,
+                                       colscale);
                     wide = true;
                 }
             }
-            if (!wide) hipLaunchKernelGGL((lu_trsm_kernel<T, 64>), tgrid, tblock, 0, s, A, lda, n, k0, k1);
-            if (use_mfma) {
+            if (!wide)
+                hipLaunchKernelGGL((lu_u12_kernel<T, 64>), ugrid, dim3(256), 0, s, A, lda, n, k0, k1 - k0, colscale);
+            if (f16) {
+                if constexpr (sizeof(T) == 4) {
+                    const int tiles = (n - k1 + kHB - 1) / kHB;
+                    hipLaunchKernelGGL(lu_gemm_f16_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, lda, n, k0, k1,
+                                       colscale);
+                }
+            } else if (use_mfma) {
                 const int rem = n - k1, bn = gemm_bn<T>();
                 hipLaunchKernelGGL(lu_gemm_mfma_kernel<T>, dim3((rem + kBM - 1) / kBM, (rem + bn - 1) / bn), dim3(256),
                                    0, s, A, lda, n, k0, k1);
@@ -827,28 +1157,14 @@ __global__ void mfma_probe_kernel(int mode, double* out64, float* out32)
 
 // ---- solves with FP32 factors: permutation + one launch per block ----------
 // A solve with the factors of the mixed mode runs once per GMRES iteration,
-// so it is built for few launches: the pivots become one permutation (once
-// per factorisation) and each 64-row block of the forward and backward sweeps
-// is one launch.  Every workgroup of the launch reads the block's 64x64
-// diagonal triangle into LDS and solves it by substitution with one wavefront
-// (lane r keeps x_r, the solved entry is broadcast with a lane shuffle), then
-// updates its share of the remaining rows; workgroup 0 stores the block's
-// solution.
+// so it is built for few launches and no serial chains: once per
+// factorisation the pivots become one permutation and the 64x64 diagonal
+// blocks of L and U are inverted (one workgroup per block); each 64-row
+// block of the forward and backward sweeps is then one launch, in which
+// every workgroup forms y = inv(D) v_block (4 lanes per row) and updates 64
+// of the remaining rows (each wavefront takes 16 of the block's columns with
+// all 16 loads in flight; the 4 partial sums meet in LDS).
 constexpr int kSB = 64;
-
-__global__ void pivots_to_perm_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ perm)
-{
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int i = 0; i < n; ++i) perm[i] = i;
-    for (int k = 0; k < n; ++k) {
-        const int p = ipiv[k];
-        if (p != k) {
-            const int t = perm[k];
-            perm[k] = perm[p];
-            perm[p] = t;
-        }
-    }
-}
 
 __global__ void permute_kernel(const double* __restrict__ x, const int* __restrict__ perm, int n,
                                double* __restrict__ y)
@@ -857,56 +1173,92 @@ __global__ void permute_kernel(const double* __restrict__ x, const int* __restri
     if (i < n) y[i] = x[perm[i]];
 }
 
+// inv[(2 b + u) 64^2 + 64 r + q] = (D_b^-1)[r][q], D_b the diagonal block b of
+// L (u = 0, unit lower) or U (u = 1).  Thread c computes column c of the
+// inverse by substitution and touches only that column: no barriers.
+template <typename T>
+__global__ void __launch_bounds__(kSB) block_inverse_kernel(const T* __restrict__ LU, int64_t lda, int n,
+                                                            T* __restrict__ inv)
+{
+    __shared__ T D[kSB][kSB + 1];
+    __shared__ T X[kSB][kSB + 1];
+    const int b = blockIdx.x, upper = blockIdx.y;
+    const int k0 = b * kSB, bs = min(kSB, n - k0);
+    const int c = threadIdx.x;
+#pragma unroll
+    for (int q0 = 0; q0 < kSB; q0 += 16) {
+        T v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = q0 + u;
+            v[u] = (c < bs && q < bs) ? LU[(int64_t)(k0 + q) * lda + k0 + c] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) D[c][q0 + u] = v[u];
+    }
+    __syncthreads();
+    if (!upper) {
+        for (int r = 0; r < kSB; ++r) {  // X[r][c] = delta_rc - sum_{c <= q < r} L[r][q] X[q][c]
+            T acc = (r == c) ? T(1) : T(0);
+            for (int q = c; q < r; ++q) acc -= D[r][q] * X[q][c];
+            X[r][c] = (r < bs && c < bs) ? acc : T(0);
+        }
+    } else {
+        for (int r = kSB - 1; r >= 0; --r) {  // X[r][c] = (delta_rc - sum_{r < q <= c} U[r][q] X[q][c]) / U[r][r]
+            T acc = (r == c) ? T(1) : T(0);
+            for (int q = r + 1; q <= c; ++q) acc -= D[r][q] * X[q][c];
+            X[r][c] = (r < bs && c < bs && r <= c) ? acc / D[r][r] : T(0);
+        }
+    }
+    __syncthreads();
+    T* out = inv + ((size_t)b * 2 + upper) * kSB * kSB;
+    for (int r = 0; r < kSB; ++r) out[r * kSB + c] = X[r][c];
+}
+
 // lower = 1: forward block (unit lower L), rows updated i >= k0+bs;
 // lower = 0: backward block (U), rows updated i < k0.
 template <typename T, int LOWER>
 __global__ void __launch_bounds__(256) sweep_block_kernel(const T* __restrict__ LU, int64_t lda, int n,
-                                                          double* __restrict__ v, double* __restrict__ out, int k0,
-                                                          int bs)
+                                                          const T* __restrict__ inv, double* __restrict__ v,
+                                                          double* __restrict__ out, int k0, int bs)
 {
-    __shared__ double D[kSB][kSB + 1];  // D[r][q] = A[k0+r, k0+q]
+    constexpr int QW = kSB / 4;  // inverse columns per lane, block columns per wavefront
+    __shared__ double vb[kSB];
     __shared__ double yk[kSB];
-    const int t = threadIdx.x;
-    for (int e = t; e < bs * bs; e += 256) {
-        const int r = e % bs, q = e / bs;
-        D[r][q] = (double)LU[(int64_t)(k0 + q) * lda + k0 + r];
+    __shared__ double part[4][64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t < kSB) vb[t] = (t < bs) ? v[k0 + t] : 0.0;
+    const T* X = inv + ((size_t)(k0 / kSB) * 2 + (LOWER ? 0 : 1)) * kSB * kSB;
+    const int r = t >> 2, qb = (t & 3) * QW;
+    T xr[QW];
+#pragma unroll
+    for (int u = 0; u < QW; ++u) xr[u] = X[r * kSB + qb + u];
+    const int i = (LOWER ? k0 + bs : 0) + blockIdx.x * 64 + lane;
+    const int rend = LOWER ? n : k0;
+    const int q0 = wave * QW;
+    T a[QW];
+    if (i < rend) {
+        const T* li = LU + (int64_t)(k0 + q0) * lda + i;
+#pragma unroll
+        for (int u = 0; u < QW; ++u) a[u] = (q0 + u < bs) ? li[(int64_t)u * lda] : T(0);
     }
     __syncthreads();
-    if (t < 64) {
-        double x = (t < bs) ? v[k0 + t] : 0.0;
-        if (LOWER) {
-            for (int q = 0; q < bs; ++q) {
-                const double xq = __shfl(x, q, 64);
-                if (t > q && t < bs) x -= D[t][q] * xq;
-            }
-        } else {
-            for (int q = bs - 1; q >= 0; --q) {
-                if (t == q) x /= D[q][q];
-                const double xq = __shfl(x, q, 64);
-                if (t < q) x -= D[t][q] * xq;
-            }
-        }
-        if (t < bs) {
-            yk[t] = x;
-            if (blockIdx.x == 0) out[k0 + t] = x;
-        }
-    }
+    double sum = 0.0;
+#pragma unroll
+    for (int u = 0; u < QW; ++u) sum += (double)xr[u] * vb[qb + u];
+    sum += __shfl_xor(sum, 1, 64);
+    sum += __shfl_xor(sum, 2, 64);
+    if ((t & 3) == 0) yk[r] = sum;
     __syncthreads();
-    const int i = LOWER ? k0 + bs + blockIdx.x * 256 + t : blockIdx.x * 256 + t;
-    if (LOWER ? (i < n) : (i < k0)) {
-        double acc = v[i];
-        const T* li = LU + (int64_t)k0 * lda + i;
-        int q = 0;
-        for (; q + 8 <= bs; q += 8) {  // eight loads in flight
-            double a[8];
+    if (blockIdx.x == 0 && t < bs) out[k0 + t] = yk[t];
+    double acc = 0.0;
+    if (i < rend) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a[u] = (double)li[(int64_t)(q + u) * lda];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) acc -= a[u] * yk[q + u];
-        }
-        for (; q < bs; ++q) acc -= (double)li[(int64_t)q * lda] * yk[q];
-        v[i] = acc;
+        for (int u = 0; u < QW; ++u) acc += (double)a[u] * yk[q0 + u];
     }
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && i < rend) v[i] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 }  // namespace
@@ -917,26 +1269,54 @@ hipError_t mfma_probe(int mode, double* out64, float* out32, hipStream_t s)
     return hipGetLastError();
 }
 
-size_t lu_workspace_bytes() { return kColMaxBlocks * (sizeof(double) + sizeof(int)) + 64; }
+// counter and pivot partials, the fp16 column scales, the panel copy (n rows
+// x 128 FP32 or 64 FP64 columns)
+size_t lu_workspace_bytes(int n) { return ws_backup_offset(n) + (size_t)n * kMaxPanel * sizeof(float); }
 
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     int block_pivot, void* ws, hipStream_t s)
+                     int block_pivot, int update_fp16, void* ws, int* redone, hipStream_t s)
 {
     if (nb < 1 || nb > (elem_bytes == 4 ? kMaxPanel : 64) || !ws || lda % 4 != 0) return hipErrorInvalidValue;
     if (elem_bytes == 8)
-        return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, ws, s);
-    return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, ws, s);
+        return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, 0, ws,
+                                   redone, s);
+    return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, update_fp16, ws,
+                              redone, s);
 }
 
-size_t lu_solve_aux_bytes(int n) { return ((sizeof(int) * (size_t)n + 255) / 256) * 256; }
+static size_t perm_bytes(int n) { return ((sizeof(int) * (size_t)n + 255) / 256) * 256; }
+
+size_t lu_solve_aux_bytes(int n, int elem_bytes)
+{
+    const size_t blocks = ((size_t)n + kSB - 1) / kSB;
+    return perm_bytes(n) + blocks * 2 * kSB * kSB * (size_t)elem_bytes;
+}
 
 hipError_t lu_prepare_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, void* aux,
                             hipStream_t s)
 {
-    (void)LU;
-    (void)elem_bytes;
-    (void)lda;
-    hipLaunchKernelGGL(pivots_to_perm_kernel, dim3(1), dim3(64), 0, s, ipiv, n, static_cast<int*>(aux));
+    // the swaps are sequential: composed on the host (n ints each way)
+    std::vector<int> piv(n), perm(n);
+    hipError_t e = hipMemcpyAsync(piv.data(), ipiv, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        const int p = piv[k];
+        if (p < 0 || p >= n) return hipErrorInvalidValue;
+        if (p != k) std::swap(perm[k], perm[p]);
+    }
+    e = hipMemcpyAsync(aux, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // perm is a local buffer
+    if (e != hipSuccess) return e;
+    const dim3 grid((n + kSB - 1) / kSB, 2);
+    void* inv = static_cast<char*>(aux) + perm_bytes(n);
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(block_inverse_kernel<double>, grid, dim3(kSB), 0, s, static_cast<const double*>(LU), lda,
+                           n, static_cast<double*>(inv));
+    else
+        hipLaunchKernelGGL(block_inverse_kernel<float>, grid, dim3(kSB), 0, s, static_cast<const float*>(LU), lda, n,
+                           static_cast<float*>(inv));
     return hipGetLastError();
 }
 
@@ -944,18 +1324,19 @@ template <typename T>
 hipError_t solve_blocks_impl(const T* LU, int64_t lda, int n, const void* aux, double* x, double* tmp, hipStream_t s)
 {
     const int* perm = static_cast<const int*>(aux);
+    const T* inv = reinterpret_cast<const T*>(static_cast<const char*>(aux) + perm_bytes(n));
     const int nb = (n + kSB - 1) / kSB;
     hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, perm, n, tmp);
     for (int b = 0; b < nb; ++b) {  // L z = P x: tmp -> x
         const int k0 = b * kSB, bs = std::min(kSB, n - k0);
         const int rows = n - k0 - bs;
-        hipLaunchKernelGGL((sweep_block_kernel<T, 1>), dim3(std::max(1, (rows + 255) / 256)), dim3(256), 0, s, LU,
-                           lda, n, tmp, x, k0, bs);
+        hipLaunchKernelGGL((sweep_block_kernel<T, 1>), dim3(std::max(1, (rows + 63) / 64)), dim3(256), 0, s, LU,
+                           lda, n, inv, tmp, x, k0, bs);
     }
     for (int b = nb - 1; b >= 0; --b) {  // U y = z: x -> tmp
         const int k0 = b * kSB, bs = std::min(kSB, n - k0);
-        hipLaunchKernelGGL((sweep_block_kernel<T, 0>), dim3(std::max(1, (k0 + 255) / 256)), dim3(256), 0, s, LU,
-                           lda, n, x, tmp, k0, bs);
+        hipLaunchKernelGGL((sweep_block_kernel<T, 0>), dim3(std::max(1, (k0 + 63) / 64)), dim3(256), 0, s, LU,
+                           lda, n, inv, x, tmp, k0, bs);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -462,11 +462,15 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
 }
 
 // LU factors of the stored matrix in d_LU / d_ipiv (once per assembled
-// system: *done is set and checked).
-int rf_sim::factorize(bool* done)
+// system: *done is set and checked).  precond_only: the factors only
+// precondition GMRES, and FP32 factors may then take the fp16 trailing
+// update (lu_update_fp16); the direct solver's refinement needs
+// FP32-accurate factors and refactors factors made that way.
+int rf_sim::factorize(bool* done, bool precond_only)
 {
-    if (*done) return RF_OK;
     const int es = lu_es;
+    const bool fp16 = precond_only && es == 4 && p.lu_update_fp16;
+    if (*done && (precond_only || !lu_fp16)) return RF_OK;
     if (!d_LU) {
         void* q = nullptr;
         if (hipMalloc(&q, (size_t)lda * n * es) != hipSuccess) {
@@ -476,7 +480,7 @@ int rf_sim::factorize(bool* done)
         allocs.push_back(q);
         d_LU = q;
         const size_t piv_bytes = (sizeof(int) * (n + 2) + 255) / 256 * 256;
-        if (hipMalloc(&q, piv_bytes + lu_workspace_bytes()) != hipSuccess) {
+        if (hipMalloc(&q, piv_bytes + lu_workspace_bytes(n)) != hipSuccess) {
             rf::set_error("cannot allocate pivots");
             return RF_ENOMEM;
         }
@@ -484,7 +488,7 @@ int rf_sim::factorize(bool* done)
         d_ipiv = static_cast<int*>(q);
         d_info = d_ipiv + n;
         d_lu_ws = static_cast<char*>(q) + piv_bytes;
-        HIP_TRY(hipMemsetAsync(d_lu_ws, 0, lu_workspace_bytes(), stream));
+        HIP_TRY(hipMemsetAsync(d_lu_ws, 0, lu_workspace_bytes(n), stream));
     }
     // panel width: 128 for FP32 factors (halves the trailing update's traffic), 64 for FP64
     const int nb_max = (es == 4) ? 128 : 64;
@@ -495,31 +499,26 @@ int rf_sim::factorize(bool* done)
         if ((rc0 = timed_pair(&ea, &eb))) return rc0;
         HIP_TRY(hipEventRecord(ea, stream));
     }
-    // the block-pivoted panel first; when its check finds that partial
-    // pivoting would have chosen another row, the copy is refactored with the
-    // column-by-column panel
+    // block-pivoted panels, each checked against partial pivoting and redone
+    // column by column when the check fails
     int info[2] = {0, 0};
-    for (int block_pivot = p.lu_column_panel ? 0 : 1; block_pivot >= 0; --block_pivot) {
-        if (es == (mixed ? 4 : 8))
-            HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
-        else  // FP32 factors of the FP64 matrix
-            HIP_TRY(launch_f64_to_f32(static_cast<const double*>(d_A), static_cast<float*>(d_LU), (size_t)lda * n,
-                                      stream));
-        HIP_TRY(hipMemsetAsync(d_info, 0, 2 * sizeof(int), stream));
-        HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, block_pivot, d_lu_ws,
-                          stream));
-        HIP_TRY(hipMemcpyAsync(h_dots, d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
-        std::memcpy(info, h_dots, 2 * sizeof(int));
-        if (!block_pivot || !info[1]) break;
-        stats.lu_panel_fallbacks += 1;
-    }
+    if (es == (mixed ? 4 : 8))
+        HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
+    else  // FP32 factors of the FP64 matrix
+        HIP_TRY(launch_f64_to_f32(static_cast<const double*>(d_A), static_cast<float*>(d_LU), (size_t)lda * n, stream));
+    HIP_TRY(hipMemsetAsync(d_info, 0, 2 * sizeof(int), stream));
+    int redone = 0;
+    HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, p.lu_column_panel ? 0 : 1,
+                      fp16 ? 1 : 0, d_lu_ws, &redone, stream));
+    stats.lu_panel_fallbacks += redone;
+    HIP_TRY(hipMemcpyAsync(h_dots, d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
     if (p.profile) {
         HIP_TRY(hipEventRecord(eb, stream));
         timed_push(ea, eb, kLU);
     }
     stats.lu_factorizations += 1;
     HIP_TRY(hipStreamSynchronize(stream));
+    std::memcpy(info, h_dots, 2 * sizeof(int));
     resolve_timed();
     if (info[0] != 0) {
         rf::set_error("LU: matrix is singular (U(" + std::to_string(info[0]) + "," + std::to_string(info[0]) + ") = 0)");
@@ -528,17 +527,18 @@ int rf_sim::factorize(bool* done)
     if (es == 4) {  // FP32 factors are applied every iteration: permutation for the block sweeps
         if (!d_lu_aux) {
             void* q = nullptr;
-            if (hipMalloc(&q, lu_solve_aux_bytes(n) + sizeof(double) * (size_t)lda) != hipSuccess) {
+            if (hipMalloc(&q, lu_solve_aux_bytes(n, es) + sizeof(double) * (size_t)lda) != hipSuccess) {
                 rf::set_error("cannot allocate the LU solve workspace");
                 return RF_ENOMEM;
             }
             allocs.push_back(q);
             d_lu_aux = q;
-            d_lu_tmp = reinterpret_cast<double*>(static_cast<char*>(q) + lu_solve_aux_bytes(n));
+            d_lu_tmp = reinterpret_cast<double*>(static_cast<char*>(q) + lu_solve_aux_bytes(n, es));
         }
         HIP_TRY(lu_prepare_solve(d_LU, es, lda, n, d_ipiv, d_lu_aux, stream));
     }
     *done = true;
+    lu_fp16 = fp16;
     return RF_OK;
 }
 
@@ -560,7 +560,7 @@ int rf_sim::solve_lu(double* be_out)
 {
     const int es = lu_es;
     {
-        int rcf = factorize(&lu_factored);
+        int rcf = factorize(&lu_factored, false);
         if (rcf) return rcf;
     }
     int rc;
@@ -729,7 +729,7 @@ int rf_sim::setup_precond(int kind)
     }
     int rc;
     if (kind == RF_PRECOND_LU) {
-        if ((rc = factorize(&lu_factored))) return rc;
+        if ((rc = factorize(&lu_factored, true))) return rc;
         prec_kind = RF_PRECOND_LU;
         prec_ready = true;
         return RF_OK;
@@ -867,12 +867,16 @@ int rf_sim::solve()
             // the first cycle the observed residual reduction per iteration
             // predicts the iterations still needed; if they exceed the budget
             // the switch happens at once.
+            // A solve that ended on the LU makes the next ones start on it
+            // (the dynamics change the matrix little per step); every
+            // kAutoRetry-th of them tries the neighbour preconditioner again.
             const int budget = std::min(maxit, p.precond_switch_iters > 0 ? p.precond_switch_iters : 2 * restart);
             const double tol = p.gmres_tol;
-            if ((rc = setup_precond(RF_PRECOND_NEIGHBOUR))) return rc;
-            if ((rc = solve_gmres(&iters, &be, &status, std::min(budget, restart)))) return rc;
+            const bool lu_first = auto_lu_run > 0 && auto_lu_run % kAutoRetry != 0;
+            if ((rc = setup_precond(lu_first ? RF_PRECOND_LU : RF_PRECOND_NEIGHBOUR))) return rc;
+            if ((rc = solve_gmres(&iters, &be, &status, lu_first ? maxit : std::min(budget, restart)))) return rc;
             const double r0 = stats.last_initial_residual;
-            if (status != 0 && iters < budget && r0 > 0.0 && be > 0.0 && be < r0) {
+            if (!lu_first && status != 0 && iters < budget && r0 > 0.0 && be > 0.0 && be < r0) {
                 const double rate = std::pow(be / r0, 1.0 / iters);
                 const double needed = std::log(tol / be) / std::log(rate);
                 if (iters + needed <= budget) {
@@ -881,16 +885,19 @@ int rf_sim::solve()
                     iters += it2;
                 }
             }
-            if (status != 0 && iters < maxit) {
+            stats.last_neighbour_iters = lu_first ? 0 : iters;
+            if (!lu_first && status != 0 && iters < maxit) {
                 if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
                 int it2 = 0;
                 if ((rc = solve_gmres(&it2, &be, &status, maxit - iters))) return rc;
                 iters += it2;
             }
             stats.last_initial_residual = r0;
+            auto_lu_run = (prec_kind == RF_PRECOND_LU) ? auto_lu_run + 1 : 0;
         } else {
             if (p.precond != RF_PRECOND_NONE && (rc = setup_precond(p.precond))) return rc;
             if ((rc = solve_gmres(&iters, &be, &status, maxit))) return rc;
+            stats.last_neighbour_iters = (p.precond == RF_PRECOND_NEIGHBOUR) ? iters : 0;
         }
         stats.last_precond = prec_kind;
         stats.last_solver = RF_SOLVER_GMRES;
@@ -1488,7 +1495,7 @@ extern "C" int rf_selftest_mfma_layout(int32_t mode, double* out64, float* out32
 }
 
 extern "C" int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t column_panel, int32_t valu,
-                              double* a, int32_t* ipiv, int32_t* info, int32_t* fallback)
+                              int32_t update_fp16, double* a, int32_t* ipiv, int32_t* info, int32_t* fallback)
 {
     const int nb_max = (elem_bytes == 4) ? 128 : 64;
     if (n < 1 || n > 65536 || (elem_bytes != 4 && elem_bytes != 8) || nb < 0 || nb > nb_max || !a || !ipiv || !info ||
@@ -1511,13 +1518,15 @@ extern "C" int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t
     void *dA = nullptr, *dP = nullptr, *dW = nullptr;
     hipError_t e = hipMalloc(&dA, host.size());
     if (e == hipSuccess) e = hipMalloc(&dP, sizeof(int) * ((size_t)n + 2));
-    if (e == hipSuccess) e = hipMalloc(&dW, lu_workspace_bytes());
+    if (e == hipSuccess) e = hipMalloc(&dW, lu_workspace_bytes(n));
     if (e == hipSuccess) e = hipMemcpy(dA, host.data(), host.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(dP, 0, sizeof(int) * ((size_t)n + 2));
-    if (e == hipSuccess) e = hipMemset(dW, 0, lu_workspace_bytes());
+    if (e == hipSuccess) e = hipMemset(dW, 0, lu_workspace_bytes(n));
     int* d_ipiv = static_cast<int*>(dP);
+    int redone = 0;
     if (e == hipSuccess)
-        e = lu_factor(dA, elem_bytes, lda, n, nb, d_ipiv, d_ipiv + n, valu ? 0 : 1, column_panel ? 0 : 1, dW, nullptr);
+        e = lu_factor(dA, elem_bytes, lda, n, nb, d_ipiv, d_ipiv + n, valu ? 0 : 1, column_panel ? 0 : 1,
+                      update_fp16, dW, &redone, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     std::vector<int> piv((size_t)n + 2);
     if (e == hipSuccess) e = hipMemcpy(piv.data(), dP, sizeof(int) * ((size_t)n + 2), hipMemcpyDeviceToHost);
@@ -1535,7 +1544,7 @@ extern "C" int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t
                                              : (double)reinterpret_cast<float*>(host.data())[c * lda + r];
     for (int i = 0; i < n; ++i) ipiv[i] = piv[i];
     *info = piv[n];
-    *fallback = piv[n + 1];
+    *fallback = redone;
     return RF_OK;
 }
 

@@ -89,6 +89,9 @@ struct rf_sim {
     void* d_lu_aux = nullptr;  // FP32 factors: permutation and diagonal-block inverses
     double* d_lu_tmp = nullptr;
     bool lu_factored = false;  // d_LU holds the factors of the current matrix
+    int auto_lu_run = 0;       // RF_PRECOND_AUTO: consecutive solves that ended on the LU
+    static constexpr int kAutoRetry = 8;
+    bool lu_fp16 = false;      // ... with fp16-rounded trailing updates (preconditioner use only)
 
     rf_stats stats{};
 
@@ -117,7 +120,7 @@ struct rf_sim {
     int norm2(const double* v, double* out);
     int solve_gmres(int* iters, double* be, int* status, int maxit);
     int solve_lu(double* be);
-    int factorize(bool* done);
+    int factorize(bool* done, bool precond_only);
     int lu_apply(double* x);
     int setup_neighbours();
     int mlp_alloc();

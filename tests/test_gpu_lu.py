@@ -53,11 +53,24 @@ def test_column_panel_matches_lapack_on_random_matrices(n, nb, valu):
     assert np.abs(LU - lu_ref).max() <= 1e-11 * np.abs(lu_ref).max()
 
 
-@pytest.mark.parametrize("n,nb", [(300, 0), (301, 40)])
-def test_block_panel_flags_matrices_that_pivot_outside_the_block(n, nb):
+@pytest.mark.parametrize("n,nb,es", [(300, 0, 8), (301, 40, 8), (300, 0, 4)])
+def test_block_panel_redoes_panels_that_pivot_outside_the_block(n, nb, es):
+    """Random matrices fail the check in every panel with rows below it:
+    each is restored from its copy and redone over whole columns, and the
+    result is partial pivoting's."""
     A = _random(n, 7)
-    _, _, info, fb = rf.selftest_lu(A, 8, nb, column_panel=0)
-    assert fb == 1
+    if es == 4:
+        A = A.astype(np.float32).astype(np.float64)
+    LU, ipiv, info, fb = rf.selftest_lu(A, es, nb, column_panel=0)
+    width = nb if nb else (128 if es == 4 else 64)
+    assert info == 0 and fb == (n - 1) // width
+    lu_ref, piv_ref = sl.lu_factor(A)
+    if es == 8:
+        np.testing.assert_array_equal(ipiv, piv_ref)
+        assert np.abs(LU - lu_ref).max() <= 1e-11 * np.abs(lu_ref).max()
+    else:
+        assert np.mean(ipiv == piv_ref) >= 0.95
+        assert _residual(A, LU, ipiv) <= 1e-5
 
 
 @pytest.mark.parametrize("n,nb,es", [(300, 0, 8), (301, 40, 8), (250, 13, 8), (515, 0, 4), (300, 100, 4),
@@ -89,11 +102,31 @@ def test_fp32_column_panel_factors_random_matrix():
     assert np.mean(ipiv == piv_ref) >= 0.95
 
 
-def test_zero_column_sets_info_or_flags_fallback():
+def test_zero_column_sets_info():
     n = 100
     A = _random(n, 5)
     A[:, 37] = 0.0
     _, _, info, fb = rf.selftest_lu(A, 8, 0, column_panel=1)
     assert info == 38 and fb == 0  # LAPACK: first k with U(k,k) = 0, 1-based
     _, _, info, fb = rf.selftest_lu(A, 8, 0, column_panel=0)
-    assert fb == 1
+    assert info == 38 and fb >= 1  # the zero pivot sends its panel to the column search
+
+
+@pytest.mark.parametrize("n,nb", [(515, 0), (300, 100), (200, 13)])
+def test_fp16_update_factors(n, nb):
+    """FP32 factors with fp16-rounded trailing-update operands: the pivots of
+    partial pivoting and a backward error of the order of fp16 rounding;
+    scaled copies of the matrix (2^-30, 2^30) give the same relative error
+    (per-column scaling of U12 keeps fp16 away from underflow and overflow)."""
+    A = _dominant_with_block_swaps(n, nb if nb else 128, 17 + n)
+    A = A.astype(np.float32).astype(np.float64)
+    _, piv_ref = sl.lu_factor(A)
+    errs = []
+    for scale in (1.0, 2.0 ** -30, 2.0 ** 30):
+        LU, ipiv, info, fb = rf.selftest_lu(A * scale, 4, nb, column_panel=0, update_fp16=1)
+        assert info == 0 and fb == 0
+        np.testing.assert_array_equal(ipiv, piv_ref)
+        errs.append(_residual(A * scale, LU, ipiv))
+    assert max(errs) <= 2e-3 and max(errs) >= 1e-7, errs  # fp16 operands, not FP32
+    LU32, _, _, _ = rf.selftest_lu(A, 4, nb, column_panel=0, update_fp16=0)
+    assert _residual(A, LU32, piv_ref) <= 1e-5

@@ -1,16 +1,19 @@
 // rf_lu.hip -- dense LU with partial pivoting on gfx950 (the direct solver,
 // LAPACK dgesv at fortran/coeffs_and_vels.f90:62,158,333,514).
 //
-// Blocked right-looking factorisation, column-major, panel width NB:
-//   for each panel [k0, k1):
+// Blocked right-looking factorisation, column-major, panel width NB, the
+// trailing update delayed to every second panel (super-panels of 2 NB):
+//   for each panel [k0, k1) (first bringing its columns up to date):
 //     either the block-pivoted panel (diagonal block in LDS, L21 = A21 U11^-1,
 //       checked against partial pivoting) or the column-by-column (Crout)
 //       panel with the pivot search over the whole column;
 //     apply the panel's row swaps to the other columns (laswp);
-//     U12 = L11^-1 A12 (unit lower triangular solve);
-//     A22 -= L21 U12 on the matrix cores (v_mfma_f64_16x16x4f64 for FP64,
-//       v_mfma_f32_16x16x4f32 for FP32 factors, or v_mfma_f32_16x16x32_f16
-//       on fp16-rounded operands for FP32 factors that only precondition).
+//     U12 = L11^-1 A12 (unit lower triangular solve, the panel's rows first
+//       brought up to date);
+//   A22 -= L21 U12 once per super-panel on the matrix cores
+//     (v_mfma_f64_16x16x4f64 for FP64, v_mfma_f32_16x16x4f32 for FP32
+//     factors, or v_mfma_f32_16x16x32_f16 on fp16-rounded operands for FP32
+//     factors that only precondition).
 // Pivots: first index of the largest |a_ik| (LAPACK idamax), so the pivot
 // sequence is dgesv's whenever the arithmetic agrees.
 #include "rf_internal.h"
@@ -469,13 +472,14 @@ __global__ void __launch_bounds__(256) lu_l21_kernel(T* A, int64_t lda, int n, i
 // U12 = L11^-1 A12 (L11 unit lower) in place for the columns [k0+nb, n) of
 // the panel rows, and the per-column powers of two for the fp16 update
 // (colscale may be null): the largest |u| of each column is taken into
-// [2^11, 2^12).  Each column is solved by the 4 lanes of a quad, NB/4 rows
+// [2^11, 2^12) (combine: the smaller of this and the scale already there,
+// for an update that also uses the U rows of an earlier panel).  Each column is solved by the 4 lanes of a quad, NB/4 rows
 // each in registers (the mirror of lu_l21_kernel): x_r is final once the
 // rows above it are applied; it is broadcast in the quad and every lane
 // subtracts L[q][r] x_r from its rows q > r.  64 columns per workgroup.
 template <typename T, int NB>
 __global__ void __launch_bounds__(256) lu_u12_kernel(T* A, int64_t lda, int n, int k0, int nb,
-                                                     float* __restrict__ colscale)
+                                                     float* __restrict__ colscale, int combine)
 {
     constexpr int CW = NB / 4;
     constexpr int CP = CW + 16 / (int)sizeof(T);
@@ -542,7 +546,8 @@ __global__ void __launch_bounds__(256) lu_u12_kernel(T* A, int64_t lda, int n, i
         m = fmaxf(m, __shfl_xor(m, 2, 64));
         if (g == 0 && live) {
             const int e = (m > 0.f && m <= 3.0e38f) ? ilogbf(m) : 11;
-            colscale[col] = ldexpf(1.f, min(60, max(-60, 11 - e)));
+            const float sc = ldexpf(1.f, min(60, max(-60, 11 - e)));
+            colscale[col] = combine ? fminf(colscale[col], sc) : sc;
         }
     }
 }
@@ -558,6 +563,11 @@ __global__ void lu_restore_kernel(T* A, int64_t lda, int n, int k0, int nb, cons
 }
 
 // ---- trailing update C -= L21 * U12 ------------------------------------
+// Every kernel updates C = A[r0:r1, c0:c1] -= A[r0:r1, ka:kb] A[ka:kb, c0:c1]
+// (the L columns and U rows of the panels [ka, kb), all inside A).
+struct GemmRange {
+    int r0, r1, c0, c1, ka, kb;
+};
 
 // VALU version (reference for the matrix-core kernels; any precision):
 // workgroup tile 64x64, K staged 16 at a time in LDS.
@@ -565,33 +575,33 @@ constexpr int kTile = 64;
 constexpr int kKc = 16;
 
 template <typename T>
-__device__ __forceinline__ void stage_tiles(const T* A, int64_t lda, int n, int rbase, int cbase, int k0, int kc,
+__device__ __forceinline__ void stage_tiles(const T* A, int64_t lda, const GemmRange& g, int rbase, int cbase, int kc,
                                             int kcount, double (&As)[kTile][kKc + 1], double (&Bs)[kKc][kTile + 1])
 {
     // As[r][kk] = L21[rbase + r, kc + kk] ; Bs[kk][c] = U12[kc + kk, cbase + c]
     for (int e = threadIdx.x; e < kTile * kKc; e += 256) {
         const int r = e % kTile, kk = e / kTile;
         const int row = rbase + r, kcol = kc + kk;
-        As[r][kk] = (row < n && kk < kcount) ? (double)A[(int64_t)kcol * lda + row] : 0.0;
+        As[r][kk] = (row < g.r1 && kk < kcount) ? (double)A[(int64_t)kcol * lda + row] : 0.0;
     }
     for (int e = threadIdx.x; e < kTile * kKc; e += 256) {
         const int kk = e % kKc, c = e / kKc;
         const int col = cbase + c, krow = kc + kk;
-        Bs[kk][c] = (col < n && kk < kcount) ? (double)A[(int64_t)col * lda + krow] : 0.0;
+        Bs[kk][c] = (col < g.c1 && kk < kcount) ? (double)A[(int64_t)col * lda + krow] : 0.0;
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, int n, int k0, int k1)
+__global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, GemmRange g)
 {
     __shared__ double As[kTile][kKc + 1];
     __shared__ double Bs[kKc][kTile + 1];
-    const int rbase = k1 + blockIdx.x * kTile, cbase = k1 + blockIdx.y * kTile;
+    const int rbase = g.r0 + blockIdx.x * kTile, cbase = g.c0 + blockIdx.y * kTile;
     const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
     double acc[4][4] = {};
-    for (int kc = k0; kc < k1; kc += kKc) {
-        const int kcount = min(kKc, k1 - kc);
-        stage_tiles<T>(A, lda, n, rbase, cbase, k0, kc, kcount, As, Bs);
+    for (int kc = g.ka; kc < g.kb; kc += kKc) {
+        const int kcount = min(kKc, g.kb - kc);
+        stage_tiles<T>(A, lda, g, rbase, cbase, kc, kcount, As, Bs);
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < kKc; ++kk) {
@@ -613,7 +623,7 @@ __global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, in
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int row = rbase + ty * 4 + u, col = cbase + tx * 4 + v;
-            if (row < n && col < n) {
+            if (row < g.r1 && col < g.c1) {
                 T* p = A + (int64_t)col * lda + row;
                 *p = (T)((double)*p - acc[u][v]);
             }
@@ -658,8 +668,8 @@ struct GemmStage {
     static constexpr int NB = kKC * BN / VL / 256;
     vec_t a[NA], b[NB];
 
-    __device__ __forceinline__ void load(const T* A, int64_t lda, int n, int rbase, int cbase, int kc, int kcount,
-                                         bool vec)
+    __device__ __forceinline__ void load(const T* A, int64_t lda, int r1, int c1, int rbase, int cbase, int kc,
+                                         int kcount, bool vec)
     {
 #pragma unroll
         for (int t = 0; t < NA; ++t) {
@@ -669,11 +679,11 @@ struct GemmStage {
             const T* src = A + (int64_t)(kc + kk) * lda + row;
             // FP64: -L21, the accumulators start at C (see the kernel)
             constexpr T sgn = (sizeof(T) == 8) ? T(-1) : T(1);
-            if (vec && kk < kcount && row + VL <= n) {
+            if (vec && kk < kcount && row + VL <= r1) {
                 a[t] = sgn * *reinterpret_cast<const vec_t*>(src);
             } else {
 #pragma unroll
-                for (int u = 0; u < VL; ++u) a[t][u] = (kk < kcount && row + u < n) ? sgn * src[u] : T(0);
+                for (int u = 0; u < VL; ++u) a[t][u] = (kk < kcount && row + u < r1) ? sgn * src[u] : T(0);
             }
         }
 #pragma unroll
@@ -682,11 +692,11 @@ struct GemmStage {
             const int c = v / (kKC / VL), kk0 = (v % (kKC / VL)) * VL;
             const int col = cbase + c;
             const T* src = A + (int64_t)col * lda + kc + kk0;
-            if (vec && col < n && kk0 + VL <= kcount) {
+            if (vec && col < c1 && kk0 + VL <= kcount) {
                 b[t] = *reinterpret_cast<const vec_t*>(src);
             } else {
 #pragma unroll
-                for (int u = 0; u < VL; ++u) b[t][u] = (col < n && kk0 + u < kcount) ? src[u] : T(0);
+                for (int u = 0; u < VL; ++u) b[t][u] = (col < c1 && kk0 + u < kcount) ? src[u] : T(0);
             }
         }
     }
@@ -709,14 +719,14 @@ struct GemmStage {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, int n, int k0, int k1)
+__global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, GemmRange g)
 {
     using V4 = typename std::conditional<sizeof(T) == 8, d4, f4>::type;
     constexpr bool kF64 = sizeof(T) == 8;
     constexpr int BN = gemm_bn<T>(), JB = BN / 32;  // JB 16-column blocks per wavefront
     __shared__ __attribute__((aligned(16))) T As[kKC][kBM + kLdsPad];
     __shared__ __attribute__((aligned(16))) T Bs[kKC][BN + kLdsPad];
-    const int rbase = k1 + blockIdx.x * kBM, cbase = k1 + blockIdx.y * BN;
+    const int rbase = g.r0 + blockIdx.x * kBM, cbase = g.c0 + blockIdx.y * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave >> 1) * 64, wc = (wave & 1) * (BN / 2);
     const int lr = lane & 15, lk = lane >> 4;
@@ -734,7 +744,7 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, in
                 for (int r = 0; r < 4; ++r) {
                     const int row = rbase + wr + 16 * i + lr;
                     const int col = cbase + wc + 16 * j + lk + 4 * r;
-                    acc[i][j][r] = (row < n && col < n) ? A[(int64_t)col * lda + row] : T(0);
+                    acc[i][j][r] = (row < g.r1 && col < g.c1) ? A[(int64_t)col * lda + row] : T(0);
                 }
             } else {
                 acc[i][j] = V4{0, 0, 0, 0};
@@ -742,13 +752,13 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, in
         }
     GemmStage<T> st;
     // 16-byte vectors need rows and K offsets that are multiples of the vector
-    const bool vec = (k0 % GemmStage<T>::VL == 0) && (k1 % GemmStage<T>::VL == 0) && (lda % GemmStage<T>::VL == 0);
-    st.load(A, lda, n, rbase, cbase, k0, min(kKC, k1 - k0), vec);
-    for (int kc = k0; kc < k1; kc += kKC) {
+    const bool vec = (g.ka % GemmStage<T>::VL == 0) && (g.r0 % GemmStage<T>::VL == 0) && (lda % GemmStage<T>::VL == 0);
+    st.load(A, lda, g.r1, g.c1, rbase, cbase, g.ka, min(kKC, g.kb - g.ka), vec);
+    for (int kc = g.ka; kc < g.kb; kc += kKC) {
         __syncthreads();  // the previous chunk's reads of As/Bs are done
         st.store(As, Bs);
         __syncthreads();
-        if (kc + kKC < k1) st.load(A, lda, n, rbase, cbase, kc + kKC, min(kKC, k1 - kc - kKC), vec);
+        if (kc + kKC < g.kb) st.load(A, lda, g.r1, g.c1, rbase, cbase, kc + kKC, min(kKC, g.kb - kc - kKC), vec);
 #pragma unroll
         for (int ks = 0; ks < kKC; ks += 4) {
             T a[4], b[JB];
@@ -776,21 +786,21 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, in
                 for (int r = 0; r < 4; ++r) {
                     const int row = rbase + wr + 16 * i + lr;
                     const int col = cbase + wc + 16 * j + lk + 4 * r;
-                    if (row < n && col < n) A[(int64_t)col * lda + row] = acc[i][j][r];
+                    if (row < g.r1 && col < g.c1) A[(int64_t)col * lda + row] = acc[i][j][r];
                 }
             } else {
                 const int row0 = rbase + wr + 16 * i + 4 * lk;
                 const int col = cbase + wc + 16 * j + lr;
-                if (col < n) {
+                if (col < g.c1) {
                     T* p = A + (int64_t)col * lda + row0;
-                    if (row0 + 3 < n && (row0 & 3) == 0) {
+                    if (row0 + 3 < g.r1 && (row0 & 3) == 0 && (lda & 3) == 0) {
                         f4 c = *reinterpret_cast<f4*>(p);
                         c -= acc[i][j];
                         *reinterpret_cast<f4*>(p) = c;
                     } else {
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
-                            if (row0 + r < n) p[r] -= acc[i][j][r];
+                            if (row0 + r < g.r1) p[r] -= acc[i][j][r];
                     }
                 }
             }
@@ -817,8 +827,8 @@ struct HalfStage {
     f4 b[2][2];  // B: column v/4, K 8*(v%4) .. +7 for v = t, t + 256
     float bs[2];
 
-    __device__ __forceinline__ void load(const float* A, int64_t lda, int n, int rbase, int cbase, int kc, int kcount,
-                                         bool vec, const float* __restrict__ colscale)
+    __device__ __forceinline__ void load(const float* A, int64_t lda, int r1, int c1, int rbase, int cbase, int kc,
+                                         int kcount, bool vec, const float* __restrict__ colscale)
     {
         const int t = threadIdx.x;
         const int r0 = rbase + 4 * (t % 32), kk0 = 4 * (t / 32);
@@ -826,11 +836,11 @@ struct HalfStage {
         for (int q = 0; q < 4; ++q) {
             const int kk = kk0 + q;
             const float* src = A + (int64_t)(kc + kk) * lda + r0;
-            if (vec && kk < kcount && r0 + 3 < n) {
+            if (vec && kk < kcount && r0 + 3 < r1) {
                 a[q] = *reinterpret_cast<const f4*>(src);
             } else {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) a[q][u] = (kk < kcount && r0 + u < n) ? src[u] : 0.f;
+                for (int u = 0; u < 4; ++u) a[q][u] = (kk < kcount && r0 + u < r1) ? src[u] : 0.f;
             }
         }
 #pragma unroll
@@ -838,15 +848,15 @@ struct HalfStage {
             const int v = t + 256 * h;
             const int col = cbase + v / 4, kb = 8 * (v % 4);
             const float* src = A + (int64_t)col * lda + kc + kb;
-            bs[h] = (col < n) ? colscale[col] : 0.f;
+            bs[h] = (col < c1) ? colscale[col] : 0.f;
 #pragma unroll
             for (int g = 0; g < 2; ++g) {
-                if (vec && col < n && kb + 4 * g + 3 < kcount) {
+                if (vec && col < c1 && kb + 4 * g + 3 < kcount) {
                     b[h][g] = *reinterpret_cast<const f4*>(src + 4 * g);
                 } else {
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
-                        b[h][g][u] = (col < n && kb + 4 * g + u < kcount) ? src[4 * g + u] : 0.f;
+                        b[h][g][u] = (col < c1 && kb + 4 * g + u < kcount) ? src[4 * g + u] : 0.f;
                 }
             }
         }
@@ -875,21 +885,21 @@ struct HalfStage {
     }
 };
 
-__global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda, int n, int k0, int k1,
+__global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda, GemmRange g,
                                                          const float* __restrict__ colscale)
 {
     __shared__ __attribute__((aligned(16))) _Float16 As[2][kHB][kHK + kHPad];
     __shared__ __attribute__((aligned(16))) _Float16 Bs[2][kHB][kHK + kHPad];
-    const int rbase = k1 + blockIdx.x * kHB, cbase = k1 + blockIdx.y * kHB;
+    const int rbase = g.r0 + blockIdx.x * kHB, cbase = g.c0 + blockIdx.y * kHB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
     const int lr = lane & 15, lg = lane >> 4;
-    const bool vec = (k0 % 4 == 0) && (k1 % 4 == 0) && (lda % 4 == 0);
+    const bool vec = (g.ka % 4 == 0) && (g.r0 % 4 == 0) && (lda % 4 == 0);
     float sc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = cbase + wc + 16 * j + lr;
-        sc[j] = (col < n) ? colscale[col] : 1.f;
+        sc[j] = (col < g.c1) ? colscale[col] : 1.f;
     }
     f4 acc[4][4];
 #pragma unroll
@@ -899,24 +909,25 @@ __global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda,
             const int row0 = rbase + wr + 16 * i + 4 * lg;
             const int col = cbase + wc + 16 * j + lr;
             const float* p = A + (int64_t)col * lda + row0;
-            if (vec && col < n && row0 + 3 < n) {
+            if (vec && col < g.c1 && row0 + 3 < g.r1) {
                 acc[i][j] = *reinterpret_cast<const f4*>(p);
             } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[i][j][r] = (col < n && row0 + r < n) ? p[r] : 0.f;
+                for (int r = 0; r < 4; ++r) acc[i][j][r] = (col < g.c1 && row0 + r < g.r1) ? p[r] : 0.f;
             }
         }
     HalfStage st;
-    st.load(A, lda, n, rbase, cbase, k0, min(kHK, k1 - k0), vec, colscale);
+    st.load(A, lda, g.r1, g.c1, rbase, cbase, g.ka, min(kHK, g.kb - g.ka), vec, colscale);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] *= sc[j];
     int buf = 0;
     st.store(As[0], Bs[0]);
-    for (int kc = k0; kc < k1; kc += kHK) {
+    for (int kc = g.ka; kc < g.kb; kc += kHK) {
         __syncthreads();  // chunk kc is in As/Bs[buf]; the other buffer's readers are done
-        if (kc + kHK < k1) st.load(A, lda, n, rbase, cbase, kc + kHK, min(kHK, k1 - kc - kHK), vec, colscale);
+        if (kc + kHK < g.kb)
+            st.load(A, lda, g.r1, g.c1, rbase, cbase, kc + kHK, min(kHK, g.kb - kc - kHK), vec, colscale);
         h8 a[4], b[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const h8*>(&As[buf][wr + 16 * i + lr][8 * lg]);
@@ -926,7 +937,7 @@ __global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda,
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
-        if (kc + kHK < k1) st.store(As[buf ^ 1], Bs[buf ^ 1]);
+        if (kc + kHK < g.kb) st.store(As[buf ^ 1], Bs[buf ^ 1]);
         buf ^= 1;
     }
 #pragma unroll
@@ -935,15 +946,15 @@ __global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda,
         for (int j = 0; j < 4; ++j) {
             const int row0 = rbase + wr + 16 * i + 4 * lg;
             const int col = cbase + wc + 16 * j + lr;
-            if (col >= n) continue;
+            if (col >= g.c1) continue;
             const f4 c = acc[i][j] * (1.f / sc[j]);
             float* p = A + (int64_t)col * lda + row0;
-            if (vec && row0 + 3 < n) {
+            if (vec && row0 + 3 < g.r1) {
                 *reinterpret_cast<f4*>(p) = c;
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    if (row0 + r < n) p[r] = c[r];
+                    if (row0 + r < g.r1) p[r] = c[r];
             }
         }
 }
@@ -1039,64 +1050,81 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
     float* colscale = f16 ? reinterpret_cast<float*>(static_cast<char*>(ws) + kWsScaleOffset) : nullptr;
     T* bak = reinterpret_cast<T*>(static_cast<char*>(ws) + ws_backup_offset(n));
     if (redone) *redone = 0;
-    for (int k0 = 0; k0 < n; k0 += nb) {
-        const int k1 = std::min(n, k0 + nb);
-        bool columns = !block_pivot;
-        if (block_pivot) {
-            block_panel<T>(A, lda, n, k0, k1, ipiv, info, bak, s);
-            // the check decides before anything else reads the panel
-            int flag = 0;
-            hipError_t e = hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) return e;
-            if (flag) {
-                hipLaunchKernelGGL(lu_restore_kernel<T>, dim3((n - k0 + 255) / 256, k1 - k0), dim3(256), 0, s, A, lda,
-                                   n, k0, k1 - k0, bak, (int64_t)n);
-                e = hipMemsetAsync(info + 1, 0, sizeof(int), s);
+    auto gemm = [&](GemmRange g) {
+        if (g.r1 <= g.r0 || g.c1 <= g.c0 || g.kb <= g.ka) return;
+        const int rows = g.r1 - g.r0, cols = g.c1 - g.c0;
+        if (f16) {
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL(lu_gemm_f16_kernel, dim3((rows + kHB - 1) / kHB, (cols + kHB - 1) / kHB), dim3(256),
+                                   0, s, A, lda, g, colscale);
+        } else if (use_mfma) {
+            const int bn = gemm_bn<T>();
+            hipLaunchKernelGGL(lu_gemm_mfma_kernel<T>, dim3((rows + kBM - 1) / kBM, (cols + bn - 1) / bn), dim3(256),
+                               0, s, A, lda, g);
+        } else {
+            hipLaunchKernelGGL(lu_gemm_valu_kernel<T>, dim3((rows + kTile - 1) / kTile, (cols + kTile - 1) / kTile),
+                               dim3(256), 0, s, A, lda, g);
+        }
+    };
+    // Super-panels of two panels: inside one, each panel's columns and rows
+    // catch up with the earlier panel (narrow updates), and the trailing
+    // matrix takes one update with K = 2 nb per super-panel, which halves its
+    // read-modify-write traffic.
+    const int sb = 2 * nb;
+    for (int K0 = 0; K0 < n; K0 += sb) {
+        const int K1 = std::min(n, K0 + sb);
+        for (int k0 = K0; k0 < K1; k0 += nb) {
+            const int k1 = std::min(K1, k0 + nb);
+            gemm({k0, n, k0, k1, K0, k0});  // the panel's columns
+            bool columns = !block_pivot;
+            if (block_pivot) {
+                block_panel<T>(A, lda, n, k0, k1, ipiv, info, bak, s);
+                // the check decides before anything else reads the panel
+                int flag = 0;
+                hipError_t e = hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
                 if (e != hipSuccess) return e;
-                columns = true;
-                if (redone) *redone += 1;
-            }
-        }
-        for (int j = k0; j < k1 && columns; ++j) {
-            const int rows = n - j;
-            const int blocks = std::max(1, std::min(kColMaxBlocks, (rows + kColThreads - 1) / kColThreads));
-            hipLaunchKernelGGL(lu_crout_kernel<T>, dim3(blocks), dim3(kColThreads), 0, s, A, lda, n, k0, k1, j, ipiv,
-                               info, pval, pidx, counter);
-        }
-        if (k1 < n && columns)
-            hipLaunchKernelGGL(lu_scale_kernel<T>, dim3((n - k1 + 255) / 256), dim3(256), 0, s, A, lda, n, k1 - 1);
-        const int outside = n - (k1 - k0);
-        if (outside > 0)
-            hipLaunchKernelGGL(lu_laswp_kernel<T>, dim3((outside + 255) / 256), dim3(256), 0, s, A, lda, n, k0, k1,
-                               ipiv);
-        if (k1 < n) {
-            const dim3 ugrid((n - k1 + 63) / 64);
-            bool wide = false;
-            if constexpr (sizeof(T) == 4) {
-                if (nb > 64) {
-                    hipLaunchKernelGGL((lu_u12_kernel<T, kMaxPanel>), ugrid, dim3(256), 0, s, A, lda, n, k0, k1 - k0,
-                                       colscale);
-                    wide = true;
+                if (flag) {
+                    hipLaunchKernelGGL(lu_restore_kernel<T>, dim3((n - k0 + 255) / 256, k1 - k0), dim3(256), 0, s, A,
+                                       lda, n, k0, k1 - k0, bak, (int64_t)n);
+                    e = hipMemsetAsync(info + 1, 0, sizeof(int), s);
+                    if (e != hipSuccess) return e;
+                    columns = true;
+                    if (redone) *redone += 1;
                 }
             }
-            if (!wide)
-                hipLaunchKernelGGL((lu_u12_kernel<T, 64>), ugrid, dim3(256), 0, s, A, lda, n, k0, k1 - k0, colscale);
-            if (f16) {
+            for (int j = k0; j < k1 && columns; ++j) {
+                const int rows = n - j;
+                const int blocks = std::max(1, std::min(kColMaxBlocks, (rows + kColThreads - 1) / kColThreads));
+                hipLaunchKernelGGL(lu_crout_kernel<T>, dim3(blocks), dim3(kColThreads), 0, s, A, lda, n, k0, k1, j,
+                                   ipiv, info, pval, pidx, counter);
+            }
+            if (k1 < n && columns)
+                hipLaunchKernelGGL(lu_scale_kernel<T>, dim3((n - k1 + 255) / 256), dim3(256), 0, s, A, lda, n, k1 - 1);
+            const int outside = n - (k1 - k0);
+            if (outside > 0)
+                hipLaunchKernelGGL(lu_laswp_kernel<T>, dim3((outside + 255) / 256), dim3(256), 0, s, A, lda, n, k0,
+                                   k1, ipiv);
+            if (k1 < n) {
+                gemm({k0, k1, k1, n, K0, k0});  // the panel's rows right of it
+                const dim3 ugrid((n - k1 + 63) / 64);
+                const int combine = (k0 > K0) ? 1 : 0;
+                bool wide = false;
                 if constexpr (sizeof(T) == 4) {
-                    const int tiles = (n - k1 + kHB - 1) / kHB;
-                    hipLaunchKernelGGL(lu_gemm_f16_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, lda, n, k0, k1,
-                                       colscale);
+                    if (nb > 64) {
+                        hipLaunchKernelGGL((lu_u12_kernel<T, kMaxPanel>), ugrid, dim3(256), 0, s, A, lda, n, k0,
+                                           k1 - k0, colscale, combine);
+                        wide = true;
+                    }
                 }
-            } else if (use_mfma) {
-                const int rem = n - k1, bn = gemm_bn<T>();
-                hipLaunchKernelGGL(lu_gemm_mfma_kernel<T>, dim3((rem + kBM - 1) / kBM, (rem + bn - 1) / bn), dim3(256),
-                                   0, s, A, lda, n, k0, k1);
-            } else {
-                const int tiles = (n - k1 + kTile - 1) / kTile;
-                hipLaunchKernelGGL(lu_gemm_valu_kernel<T>, dim3(tiles, tiles), dim3(256), 0, s, A, lda, n, k0, k1);
+                if (!wide)
+                    hipLaunchKernelGGL((lu_u12_kernel<T, 64>), ugrid, dim3(256), 0, s, A, lda, n, k0, k1 - k0, colscale,
+                                       combine);
             }
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
         }
+        gemm({K1, n, K1, n, K0, K1});  // the trailing matrix
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

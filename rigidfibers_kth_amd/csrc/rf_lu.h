@@ -11,10 +11,11 @@ namespace rf {
 // float otherwise); ipiv[k] = row swapped with k (0-based); info[0] = k+1 if
 // U(k,k) == 0.  use_mfma selects the matrix-core trailing update; panel
 // width nb <= 128 for float, <= 64 for double.  ws:
-// lu_workspace_bytes(n) of device memory, zero on entry (the column
-// kernels' counter in it is left zero).
+// lu_workspace_bytes(n, update_fp16) of device memory, zero on entry (the
+// column kernels' counter in it is left zero).
 // update_fp16 = 1 (float, use_mfma): trailing updates with fp16-rounded
-// operands and FP32 accumulation, for factors that precondition.
+// operands and FP32 accumulation, for factors that precondition; the
+// workspace must then have been sized with update_fp16 = 1.
 // block_pivot = 1 searches each panel's pivots inside its diagonal block
 // first and checks the result against partial pivoting (every multiplier
 // |l| <= 1, no zero pivot); a panel that fails is restored and factorised
@@ -22,7 +23,7 @@ namespace rf {
 // are partial pivoting's either way.  *redone (may be null) receives the
 // number of panels redone.  Block pivoting synchronises the stream once per
 // panel.  info[0..1] are zero on entry.
-size_t lu_workspace_bytes(int n);
+size_t lu_workspace_bytes(int n, int update_fp16);
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
                      int block_pivot, int update_fp16, void* ws, int* redone, hipStream_t s);
 // x <- U^-1 L^-1 P x with FP64 vectors (substitution, two launches per

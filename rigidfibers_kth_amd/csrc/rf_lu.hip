@@ -230,14 +230,14 @@ __global__ void lu_laswp_kernel(T* A, int64_t lda, int n, int k0, int k1, const 
 // systems pass (near contacts make a few fail), and a passing panel costs
 // two launches instead of one per column.
 
-// Reads the nb x nb block of A at (k0, k0) (nb <= NB) with 16 loads in
-// flight per thread and hands element (r, c) to f(r, c, v).  256 threads:
-// thread t covers row t % NB of the columns t / NB + (256 / NB) u, so each
-// load instruction reads consecutive rows of one column.
-template <typename T, int NB, typename F>
+// Reads the nb x nb block of A at (k0, k0) (nb <= NB) with up to 16 loads
+// in flight per thread and hands element (r, c) to f(r, c, v).  THREADS
+// threads: thread t covers row t % NB of the columns t / NB + (THREADS / NB) u,
+// so each load instruction reads consecutive rows of one column.
+template <typename T, int NB, int THREADS = 256, typename F>
 __device__ __forceinline__ void load_block(const T* A, int64_t lda, int k0, int nb, F&& f)
 {
-    constexpr int H = 256 / NB, PER = NB / H, BATCH = 16;
+    constexpr int H = THREADS / NB, PER = NB / H, BATCH = PER < 16 ? PER : 16;
     static_assert(PER % BATCH == 0, "columns per thread");
     const int r = threadIdx.x % NB, c0 = threadIdx.x / NB;
 #pragma unroll
@@ -258,16 +258,19 @@ __device__ __forceinline__ void load_block(const T* A, int64_t lda, int k0, int 
 
 // Diagonal block [k0, k0+nb) factorised with pivot search inside the block;
 // ipiv[k0..k0+nb) as LAPACK (0-based global rows); bak receives the block's
-// original values.  One workgroup of 256: thread t owns column t % NB and the
-// rows r = t / NB (mod 256 / NB).  Two barriers per column: (A) the pivot
+// original values.  One workgroup of kDiagThreads: thread t owns column
+// t % NB and the rows r = t / NB (mod kDiagThreads / NB).  Two barriers per column: (A) the pivot
 // row swap and the scaling of column j, (B) the rank-1 update, in which the
 // owners of column j+1 also find their partial argmax for the next column.
 // The update loads 8 rows before it stores any (LDS latency overlapped).
+constexpr int kDiagThreads = 1024;
+constexpr int kSuperPanels = 8;  // panels per trailing update
+
 template <typename T, int NB>
-__global__ void __launch_bounds__(256) lu_diag_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv, int* info,
-                                                      T* __restrict__ bak, int64_t ldb)
+__global__ void __launch_bounds__(kDiagThreads) lu_diag_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv,
+                                                               int* info, T* __restrict__ bak, int64_t ldb)
 {
-    constexpr int H = 256 / NB;  // row phases
+    constexpr int H = kDiagThreads / NB;  // row phases
     __shared__ T S[NB][NB + 1];  // S[r][c] = A[k0 + r, k0 + c]
     __shared__ double cand_v[H];
     __shared__ int cand_i[H];
@@ -277,7 +280,7 @@ __global__ void __launch_bounds__(256) lu_diag_kernel(T* A, int64_t lda, int k0,
     __shared__ int zero_l;     // before a barrier would make every barrier wait for it
     const int t = threadIdx.x, c = t % NB, h = t / NB;
     if (t == 0) zero_l = 0;
-    load_block<T, NB>(A, lda, k0, nb, [&](int r, int cc, T v) {
+    load_block<T, NB, kDiagThreads>(A, lda, k0, nb, [&](int r, int cc, T v) {
         S[r][cc] = v;
         bak[(int64_t)cc * ldb + r] = v;
     });
@@ -319,7 +322,7 @@ __global__ void __launch_bounds__(256) lu_diag_kernel(T* A, int64_t lda, int k0,
             S[j][j] = pv;
         }
         const T inv = (pv != T(0)) ? T(1) / pv : T(0);
-        if (p != j && h == 0 && c != j && c < nb) {
+        if (p != j && h == 0 && c != j && c < nb) {  // one thread per column
             const T x = S[j][c];
             S[j][c] = S[p][c];
             S[p][c] = x;
@@ -373,7 +376,7 @@ __global__ void __launch_bounds__(256) lu_diag_kernel(T* A, int64_t lda, int k0,
         }
         __syncthreads();
     }
-    for (int e = t; e < nb * nb; e += 256) {
+    for (int e = t; e < nb * nb; e += kDiagThreads) {
         const int r = e % nb, cc = e / nb;
         A[(int64_t)(k0 + cc) * lda + k0 + r] = S[r][cc];
     }
@@ -885,21 +888,42 @@ struct HalfStage {
     }
 };
 
-__global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda, GemmRange g,
-                                                         const float* __restrict__ colscale)
+// Staging from the FP32 matrix (conversion and scaling on the way), one K
+// chunk of [ka, kb) per call.
+struct HalfFromMatrix {
+    HalfStage st;
+    const float* A;
+    int64_t lda;
+    int r1, c1, rbase, cbase, ka, kb;
+    bool vec;
+    const float* colscale;
+    __device__ __forceinline__ void load(int chunk)
+    {
+        const int kc = ka + chunk * kHK;
+        st.load(A, lda, r1, c1, rbase, cbase, kc, min(kHK, kb - kc), vec, colscale);
+    }
+    __device__ __forceinline__ void store(_Float16 (&As)[kHB][kHK + kHPad], _Float16 (&Bs)[kHB][kHK + kHPad]) const
+    {
+        st.store(As, Bs);
+    }
+};
+
+// One 128 x 128 tile C[rbase.., cbase..] (rows < r1, columns < c1) of the
+// fp16 update over nchunks K chunks from the staging object.
+template <class Stage>
+__device__ __forceinline__ void f16_update_tile(float* A, int64_t lda, int rbase, int cbase, int r1, int c1, bool vec,
+                                                int nchunks, Stage& st, const float* __restrict__ colscale,
+                                                _Float16 (&As)[2][kHB][kHK + kHPad],
+                                                _Float16 (&Bs)[2][kHB][kHK + kHPad])
 {
-    __shared__ __attribute__((aligned(16))) _Float16 As[2][kHB][kHK + kHPad];
-    __shared__ __attribute__((aligned(16))) _Float16 Bs[2][kHB][kHK + kHPad];
-    const int rbase = g.r0 + blockIdx.x * kHB, cbase = g.c0 + blockIdx.y * kHB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
     const int lr = lane & 15, lg = lane >> 4;
-    const bool vec = (g.ka % 4 == 0) && (g.r0 % 4 == 0) && (lda % 4 == 0);
     float sc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int col = cbase + wc + 16 * j + lr;
-        sc[j] = (col < g.c1) ? colscale[col] : 1.f;
+        sc[j] = (col < c1) ? colscale[col] : 1.f;
     }
     f4 acc[4][4];
 #pragma unroll
@@ -909,25 +933,23 @@ __global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda,
             const int row0 = rbase + wr + 16 * i + 4 * lg;
             const int col = cbase + wc + 16 * j + lr;
             const float* p = A + (int64_t)col * lda + row0;
-            if (vec && col < g.c1 && row0 + 3 < g.r1) {
+            if (vec && col < c1 && row0 + 3 < r1) {
                 acc[i][j] = *reinterpret_cast<const f4*>(p);
             } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[i][j][r] = (col < g.c1 && row0 + r < g.r1) ? p[r] : 0.f;
+                for (int r = 0; r < 4; ++r) acc[i][j][r] = (col < c1 && row0 + r < r1) ? p[r] : 0.f;
             }
         }
-    HalfStage st;
-    st.load(A, lda, g.r1, g.c1, rbase, cbase, g.ka, min(kHK, g.kb - g.ka), vec, colscale);
+    st.load(0);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] *= sc[j];
     int buf = 0;
     st.store(As[0], Bs[0]);
-    for (int kc = g.ka; kc < g.kb; kc += kHK) {
-        __syncthreads();  // chunk kc is in As/Bs[buf]; the other buffer's readers are done
-        if (kc + kHK < g.kb)
-            st.load(A, lda, g.r1, g.c1, rbase, cbase, kc + kHK, min(kHK, g.kb - kc - kHK), vec, colscale);
+    for (int ch = 0; ch < nchunks; ++ch) {
+        __syncthreads();  // chunk ch is in As/Bs[buf]; the other buffer's readers are done
+        if (ch + 1 < nchunks) st.load(ch + 1);
         h8 a[4], b[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const h8*>(&As[buf][wr + 16 * i + lr][8 * lg]);
@@ -937,7 +959,7 @@ __global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda,
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
-        if (kc + kHK < g.kb) st.store(As[buf ^ 1], Bs[buf ^ 1]);
+        if (ch + 1 < nchunks) st.store(As[buf ^ 1], Bs[buf ^ 1]);
         buf ^= 1;
     }
 #pragma unroll
@@ -946,17 +968,154 @@ __global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda,
         for (int j = 0; j < 4; ++j) {
             const int row0 = rbase + wr + 16 * i + 4 * lg;
             const int col = cbase + wc + 16 * j + lr;
-            if (col >= g.c1) continue;
+            if (col >= c1) continue;
             const f4 c = acc[i][j] * (1.f / sc[j]);
             float* p = A + (int64_t)col * lda + row0;
-            if (vec && row0 + 3 < g.r1) {
+            if (vec && row0 + 3 < r1) {
                 *reinterpret_cast<f4*>(p) = c;
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    if (row0 + r < g.r1) p[r] = c[r];
+                    if (row0 + r < r1) p[r] = c[r];
             }
         }
+}
+
+__global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda, GemmRange g,
+                                                         const float* __restrict__ colscale)
+{
+    __shared__ __attribute__((aligned(16))) _Float16 As[2][kHB][kHK + kHPad];
+    __shared__ __attribute__((aligned(16))) _Float16 Bs[2][kHB][kHK + kHPad];
+    HalfFromMatrix st;
+    st.A = A;
+    st.lda = lda;
+    st.r1 = g.r1;
+    st.c1 = g.c1;
+    st.rbase = g.r0 + blockIdx.x * kHB;
+    st.cbase = g.c0 + blockIdx.y * kHB;
+    st.ka = g.ka;
+    st.kb = g.kb;
+    st.vec = (g.ka % 4 == 0) && (g.r0 % 4 == 0) && (lda % 4 == 0);
+    st.colscale = colscale;
+    f16_update_tile(A, lda, st.rbase, st.cbase, g.r1, g.c1, st.vec, (g.kb - g.ka + kHK - 1) / kHK, st, colscale, As,
+                    Bs);
+}
+
+// ---- packed fp16 operands of a super-panel's trailing update ---------------
+// After the panels [K0, K1) of a super-panel: Lp[i][k] = -L[K1 + i][K0 + k]
+// and Up[j][k] = U[K0 + k][K1 + j] s_{K1+j} as fp16 with K contiguous (row
+// stride kPackK, zero for K1 - K0 <= k < the width rounded to a chunk), so the
+// update's staging is one 16-byte copy per row segment.
+constexpr int kPackK = kSuperPanels * kMaxPanel;
+
+// L: a 64-row x 64-K tile per workgroup, transposed through LDS.
+__global__ void __launch_bounds__(256) pack_l_kernel(const float* __restrict__ A, int64_t lda, int n, int K0, int K1,
+                                                     int kwp, _Float16* __restrict__ Lp)
+{
+    __shared__ float tile[64][65];
+    const int t = threadIdx.x;
+    const int r0 = K1 + blockIdx.x * 64, kt = blockIdx.y * 64;
+    const int kw = K1 - K0;
+    {
+        const int row = r0 + (t & 63), kq = t >> 6;
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = kt + kq + 4 * u;
+            v[u] = (row < n && k < kw) ? A[(int64_t)(K0 + k) * lda + row] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) tile[kq + 4 * u][t & 63] = v[u];
+    }
+    __syncthreads();
+    const int rl = t >> 2, seg = (t & 3) * 16;
+    const int row = r0 + rl;
+    if (row >= n || kt + seg >= kwp) return;
+    h8 w0, w1;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        w0[v] = (_Float16)(-tile[seg + v][rl]);
+        w1[v] = (_Float16)(-tile[seg + 8 + v][rl]);
+    }
+    _Float16* dst = Lp + (size_t)(row - K1) * kPackK + kt + seg;
+    *reinterpret_cast<h8*>(dst) = w0;
+    *reinterpret_cast<h8*>(dst + 8) = w1;
+}
+
+// U: one thread per column and 8-K segment (two 16-byte reads, one write).
+__global__ void __launch_bounds__(256) pack_u_kernel(const float* __restrict__ A, int64_t lda, int n, int K0, int K1,
+                                                     int kwp, const float* __restrict__ colscale,
+                                                     _Float16* __restrict__ Up)
+{
+    const int segs = kwp / 8;
+    const int64_t gidx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int col = K1 + (int)(gidx / segs), seg = (int)(gidx % segs);
+    if (col >= n) return;
+    const int kw = K1 - K0, k0 = seg * 8;
+    const float sc = colscale[col];
+    const float* src = A + (int64_t)col * lda + K0 + k0;
+    h8 w;
+    if (k0 + 8 <= kw && (lda % 4 == 0) && (K0 % 4 == 0)) {
+        const f4 x0 = *reinterpret_cast<const f4*>(src), x1 = *reinterpret_cast<const f4*>(src + 4);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            w[v] = (_Float16)(x0[v] * sc);
+            w[4 + v] = (_Float16)(x1[v] * sc);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) w[v] = (_Float16)((k0 + v < kw) ? src[v] * sc : 0.f);
+    }
+    *reinterpret_cast<h8*>(Up + (size_t)(col - K1) * kPackK + k0) = w;
+}
+
+// Staging from the packed operands: per chunk each thread copies two
+// 16-byte segments of A rows and two of B columns.
+struct HalfFromPacked {
+    h8 a[2], b[2];
+    const _Float16* Lp;
+    const _Float16* Up;
+    int ri, cj, rn, cn;  // first packed row / column of the tile, packed rows / columns
+    __device__ __forceinline__ void load(int chunk)
+    {
+        const int kc = chunk * kHK;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int v = threadIdx.x + 256 * h;
+            const int row = v >> 2, seg = (v & 3) * 8;
+            a[h] = (ri + row < rn) ? *reinterpret_cast<const h8*>(Lp + (size_t)(ri + row) * kPackK + kc + seg) : h8{};
+            b[h] = (cj + row < cn) ? *reinterpret_cast<const h8*>(Up + (size_t)(cj + row) * kPackK + kc + seg) : h8{};
+        }
+    }
+    __device__ __forceinline__ void store(_Float16 (&As)[kHB][kHK + kHPad], _Float16 (&Bs)[kHB][kHK + kHPad]) const
+    {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int v = threadIdx.x + 256 * h;
+            *reinterpret_cast<h8*>(&As[v >> 2][(v & 3) * 8]) = a[h];
+            *reinterpret_cast<h8*>(&Bs[v >> 2][(v & 3) * 8]) = b[h];
+        }
+    }
+};
+
+// The trailing update of a super-panel from the packed operands: rows and
+// columns [K1, n), kwp / 32 chunks of K.
+__global__ void __launch_bounds__(256) lu_gemm_f16p_kernel(float* A, int64_t lda, int n, int K1, int kwp,
+                                                          const _Float16* __restrict__ Lp,
+                                                          const _Float16* __restrict__ Up,
+                                                          const float* __restrict__ colscale)
+{
+    __shared__ __attribute__((aligned(16))) _Float16 As[2][kHB][kHK + kHPad];
+    __shared__ __attribute__((aligned(16))) _Float16 Bs[2][kHB][kHK + kHPad];
+    HalfFromPacked st;
+    st.Lp = Lp;
+    st.Up = Up;
+    st.ri = blockIdx.x * kHB;
+    st.cj = blockIdx.y * kHB;
+    st.rn = n - K1;
+    st.cn = n - K1;
+    const bool vec = (K1 % 4 == 0) && (lda % 4 == 0);
+    f16_update_tile(A, lda, K1 + st.ri, K1 + st.cj, n, n, vec, kwp / kHK, st, colscale, As, Bs);
 }
 
 // ---- solves with the factors (FP64 vectors) ------------------------------
@@ -1022,8 +1181,8 @@ void block_panel(T* A, int64_t lda, int n, int k0, int k1, int* ipiv, int* info,
     const dim3 grid((n - k1 + 63) / 64);
     if constexpr (sizeof(T) == 4) {
         if (nb > 64) {
-            hipLaunchKernelGGL((lu_diag_kernel<T, kMaxPanel>), dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info,
-                               bak, (int64_t)n);
+            hipLaunchKernelGGL((lu_diag_kernel<T, kMaxPanel>), dim3(1), dim3(kDiagThreads), 0, s, A, lda, k0, nb,
+                               ipiv, info, bak, (int64_t)n);
             if (k1 < n)
                 hipLaunchKernelGGL((lu_l21_kernel<T, kMaxPanel>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info, bak,
                                    (int64_t)n);
@@ -1031,13 +1190,15 @@ void block_panel(T* A, int64_t lda, int n, int k0, int k1, int* ipiv, int* info,
         }
     }
     // FP64 panels are at most 64 wide
-    hipLaunchKernelGGL((lu_diag_kernel<T, 64>), dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info, bak, (int64_t)n);
+    hipLaunchKernelGGL((lu_diag_kernel<T, 64>), dim3(1), dim3(kDiagThreads), 0, s, A, lda, k0, nb, ipiv, info, bak,
+                       (int64_t)n);
     if (k1 < n)
         hipLaunchKernelGGL((lu_l21_kernel<T, 64>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info, bak, (int64_t)n);
 }
 
 constexpr size_t kWsScaleOffset = kColMaxBlocks * (sizeof(double) + sizeof(int)) + 64;
 size_t ws_backup_offset(int n) { return kWsScaleOffset + (sizeof(float) * (size_t)n + 255) / 256 * 256; }
+size_t ws_pack_offset(int n) { return ws_backup_offset(n) + (size_t)n * kMaxPanel * sizeof(float); }
 
 template <typename T>
 hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, int block_pivot,
@@ -1049,6 +1210,8 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
     const bool f16 = (sizeof(T) == 4) && use_mfma && update_fp16;
     float* colscale = f16 ? reinterpret_cast<float*>(static_cast<char*>(ws) + kWsScaleOffset) : nullptr;
     T* bak = reinterpret_cast<T*>(static_cast<char*>(ws) + ws_backup_offset(n));
+    _Float16* lpack = reinterpret_cast<_Float16*>(static_cast<char*>(ws) + ws_pack_offset(n));
+    _Float16* upack = lpack + (size_t)n * kPackK;
     if (redone) *redone = 0;
     auto gemm = [&](GemmRange g) {
         if (g.r1 <= g.r0 || g.c1 <= g.c0 || g.kb <= g.ka) return;
@@ -1070,7 +1233,7 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
     // catch up with the earlier panel (narrow updates), and the trailing
     // matrix takes one update with K = 2 nb per super-panel, which halves its
     // read-modify-write traffic.
-    const int sb = 2 * nb;
+    const int sb = kSuperPanels * nb;
     for (int K0 = 0; K0 < n; K0 += sb) {
         const int K1 = std::min(n, K0 + sb);
         for (int k0 = K0; k0 < K1; k0 += nb) {
@@ -1124,7 +1287,21 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
-        gemm({K1, n, K1, n, K0, K1});  // the trailing matrix
+        if (f16 && K1 < n) {  // the trailing matrix from packed fp16 operands
+            if constexpr (sizeof(T) == 4) {
+                const int rem = n - K1, kwp = (K1 - K0 + kHK - 1) / kHK * kHK;
+                hipLaunchKernelGGL(pack_l_kernel, dim3((rem + 63) / 64, (kwp + 63) / 64), dim3(256), 0, s, A, lda, n,
+                                   K0, K1, kwp, lpack);
+                const int64_t units = (int64_t)rem * (kwp / 8);
+                hipLaunchKernelGGL(pack_u_kernel, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, s, A, lda, n,
+                                   K0, K1, kwp, colscale, upack);
+                const int tiles = (rem + kHB - 1) / kHB;
+                hipLaunchKernelGGL(lu_gemm_f16p_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, lda, n, K1, kwp, lpack,
+                                   upack, colscale);
+            }
+        } else {
+            gemm({K1, n, K1, n, K0, K1});  // the trailing matrix
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -1299,7 +1476,10 @@ hipError_t mfma_probe(int mode, double* out64, float* out32, hipStream_t s)
 
 // counter and pivot partials, the fp16 column scales, the panel copy (n rows
 // x 128 FP32 or 64 FP64 columns)
-size_t lu_workspace_bytes(int n) { return ws_backup_offset(n) + (size_t)n * kMaxPanel * sizeof(float); }
+size_t lu_workspace_bytes(int n, int update_fp16)
+{
+    return ws_pack_offset(n) + (update_fp16 ? 2 * (size_t)n * kPackK * sizeof(_Float16) : 0);
+}
 
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
                      int block_pivot, int update_fp16, void* ws, int* redone, hipStream_t s)

@@ -472,6 +472,7 @@ int rf_sim::factorize(bool* done, bool precond_only)
     const bool fp16 = precond_only && es == 4 && p.lu_update_fp16;
     if (*done && (precond_only || !lu_fp16)) return RF_OK;
     if (!d_LU) {
+        const int ws_fp16 = (es == 4 && p.lu_update_fp16) ? 1 : 0;
         void* q = nullptr;
         if (hipMalloc(&q, (size_t)lda * n * es) != hipSuccess) {
             rf::set_error("cannot allocate the LU workspace");
@@ -480,7 +481,7 @@ int rf_sim::factorize(bool* done, bool precond_only)
         allocs.push_back(q);
         d_LU = q;
         const size_t piv_bytes = (sizeof(int) * (n + 2) + 255) / 256 * 256;
-        if (hipMalloc(&q, piv_bytes + lu_workspace_bytes(n)) != hipSuccess) {
+        if (hipMalloc(&q, piv_bytes + lu_workspace_bytes(n, ws_fp16)) != hipSuccess) {
             rf::set_error("cannot allocate pivots");
             return RF_ENOMEM;
         }
@@ -488,7 +489,7 @@ int rf_sim::factorize(bool* done, bool precond_only)
         d_ipiv = static_cast<int*>(q);
         d_info = d_ipiv + n;
         d_lu_ws = static_cast<char*>(q) + piv_bytes;
-        HIP_TRY(hipMemsetAsync(d_lu_ws, 0, lu_workspace_bytes(n), stream));
+        HIP_TRY(hipMemsetAsync(d_lu_ws, 0, lu_workspace_bytes(n, ws_fp16), stream));
     }
     // panel width: 128 for FP32 factors (halves the trailing update's traffic), 64 for FP64
     const int nb_max = (es == 4) ? 128 : 64;
@@ -1518,10 +1519,10 @@ extern "C" int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t
     void *dA = nullptr, *dP = nullptr, *dW = nullptr;
     hipError_t e = hipMalloc(&dA, host.size());
     if (e == hipSuccess) e = hipMalloc(&dP, sizeof(int) * ((size_t)n + 2));
-    if (e == hipSuccess) e = hipMalloc(&dW, lu_workspace_bytes(n));
+    if (e == hipSuccess) e = hipMalloc(&dW, lu_workspace_bytes(n, update_fp16));
     if (e == hipSuccess) e = hipMemcpy(dA, host.data(), host.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(dP, 0, sizeof(int) * ((size_t)n + 2));
-    if (e == hipSuccess) e = hipMemset(dW, 0, lu_workspace_bytes(n));
+    if (e == hipSuccess) e = hipMemset(dW, 0, lu_workspace_bytes(n, update_fp16));
     int* d_ipiv = static_cast<int*>(dP);
     int redone = 0;
     if (e == hipSuccess)

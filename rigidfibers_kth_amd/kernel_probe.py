@@ -2,7 +2,7 @@
 
     python3 -m rigidfibers_kth_amd.kernel_probe gemv --fibers 1000 --reps 20
     python3 -m rigidfibers_kth_amd.kernel_probe copy --bytes 1073741824 --reps 5
-    python3 -m rigidfibers_kth_amd.kernel_probe lu --fibers 1000 [--fp32]
+    python3 -m rigidfibers_kth_amd.kernel_probe lu --fibers 1000 [--fp32 [--fp16]] [--column-panel]
     python3 -m rigidfibers_kth_amd.kernel_probe assemble --fibers 1000 --reps 5
 
 The layouts are the seeded ones of the benchmark (volume fraction 1e-2 for
@@ -10,7 +10,9 @@ N <= 2000, 5e-2 above).  Each run prints what it launched, so a profile of
 the command can be read against it: `gemv` launches the system-matrix GEMV
 (rf_matvec) `reps` times after one assembly, `copy` streams a buffer through
 the plain copy kernel (HBM byte-counter calibration), `lu` factorises the
-assembled matrix once, `assemble` assembles `reps` times.
+assembled matrix once (as the direct solver; with --fp16 as the GMRES
+preconditioner, whose FP32 factors take the fp16-operand trailing update),
+`assemble` assembles `reps` times.
 """
 import argparse
 import sys
@@ -18,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import PREC_FP64, PREC_MIXED, Configuration, Simulation, default_params, selftest_copy
+from . import PREC_FP64, PREC_MIXED, PRECOND_LU, Configuration, Simulation, default_params, selftest_copy
 from .layouts import layout
 
 
@@ -35,6 +37,9 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--fp32", action="store_true", help="FP32 matrix (gemv, assemble) or FP32 factors (lu)")
+    ap.add_argument("--fp16", action="store_true", help="lu: FP32 factors with fp16-operand trailing updates, "
+                    "factorised as the preconditioner of one GMRES solve")
+    ap.add_argument("--column-panel", action="store_true", help="lu: pivot search over whole columns in every panel")
     a = ap.parse_args(argv)
     if a.kernel == "copy":
         ms = selftest_copy(a.bytes, a.reps)
@@ -44,7 +49,11 @@ def main(argv=None):
     prec = PREC_MIXED if (a.fp32 and a.kernel != "lu") else PREC_FP64
     kw = dict(precision=prec, profile=1)
     if a.kernel == "lu":
-        kw.update(use_direct_solver=1, lu_factor_fp32=int(a.fp32))
+        kw.update(lu_factor_fp32=int(a.fp32 or a.fp16), lu_column_panel=int(a.column_panel))
+        if a.fp16:
+            kw.update(precond=PRECOND_LU, lu_update_fp16=1, gmres_restart=30, gmres_max_iters=300, gmres_tol=1e-10)
+        else:
+            kw.update(use_direct_solver=1)
     with _sim(a.fibers, **kw) as sim:
         sim.assemble_system()
         sim.synchronize()
@@ -69,8 +78,10 @@ def main(argv=None):
             sim.solve_system()
             sim.synchronize()
             st = sim.stats()
-            print("lu: n=%d, factors %s, %.1f ms, backward error %.2e"
-                  % (sim.n, "fp32" if a.fp32 else "fp64", st.lu_ms_total, st.last_backward_error), flush=True)
+            kind = "fp32, fp16-operand updates" if a.fp16 else ("fp32" if a.fp32 else "fp64")
+            print("lu: n=%d, factors %s, %.1f ms, panels redone %d, GMRES iterations %d, backward error %.2e"
+                  % (sim.n, kind, st.lu_ms_total, st.lu_panel_fallbacks, st.last_gmres_iters, st.last_backward_error),
+                  flush=True)
     return 0
 
 

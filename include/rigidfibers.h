@@ -256,7 +256,7 @@ int rf_selftest_copy(int64_t bytes, int32_t reps, double *ms);
  * factorisation (the dgesv factorisation of fortran/coeffs_and_vels.f90:62,158)
  * of an arbitrary n x n column-major matrix a, overwritten with L\U (unit L
  * below the diagonal).  elem_bytes 8: FP64; 4: a rounded to FP32 and
- * factorised in FP32.  nb: panel width (0 = 64 in FP64, 128 in FP32).
+ * factorised in FP32.  nb: panel width (0 = 64; at most 64 in FP64, 128 in FP32).
  * column_panel 1: pivot search over whole columns; 0: inside each panel's
  * diagonal block first, and a panel where partial pivoting would have
  * chosen another row is redone column by column (*fallback = number of such

@@ -70,6 +70,8 @@ hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double
                        const double* b, int mode, hipStream_t s, hipEvent_t ev0 = nullptr,
                        hipEvent_t ev1 = nullptr);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// bytes (a multiple of 4, at most 64 KiB) into coherent pinned host memory
+hipError_t launch_to_host(const void* src, void* host_dst, size_t bytes, hipStream_t s);
 hipError_t launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
 
 // Deterministic reductions / vector kernels (rf_linalg.hip).
@@ -85,9 +87,11 @@ hipError_t launch_orth_update(const double* V, int64_t ldv, int k, const double*
                               hipStream_t s);
 hipError_t launch_scale_copy(const double* w, const double* scale_num, int scale_idx, int n, double* v,
                              hipStream_t s);
-// xc = x + V[:, :k] y
-hipError_t launch_vy_plus_x(const double* V, int64_t ldv, int k, const double* y, const double* x, int n,
+// xc = x + V[:, :k] y, y in host memory (passed as kernel arguments)
+hipError_t launch_vy_plus_x(const double* V, int64_t ldv, int k, const double* y_host, const double* x, int n,
                             double* xc, hipStream_t s);
+// dst[0:n] = src[0:n] on the device (a kernel: no runtime copy packet)
+hipError_t launch_copy_vec(const double* src, double* dst, int n, hipStream_t s);
 int multidot_blocks(int n);
 
 // Velocity epilogue, advance (rf_step.hip).

@@ -98,23 +98,32 @@ __global__ void __launch_bounds__(kGemvThreads) gemv_parts_kernel(int cols, int6
     }
 }
 
-// y[i] = sum_s parts[s][i]  (mode 0)  or  b[i] - sum_s parts[s][i]  (mode 1)
+// y[i] = sum_s parts[s][i]  (mode 0)  or  b[i] - sum_s parts[s][i]  (mode 1).
+// Eight lanes per row: lane q sums the splits q, q+8, ... (all its loads in
+// flight together), and the eight partial sums meet in a fixed xor tree, so
+// the result is deterministic.
+constexpr int kSplitLanes = 8;
+
 __global__ void reduce_splits_kernel(const double* __restrict__ parts, int splits, int64_t stride, int n,
                                      const double* __restrict__ b, int mode, double* __restrict__ y)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(t / kSplitLanes), q = (int)(t % kSplitLanes);
     double s = 0.0;
-    int k = 0;
-    for (; k + 8 <= splits; k += 8) {  // eight loads in flight, summed in split order
-        double v[8];
+    if (i < n) {
+        int k = q;
+        for (; k + 3 * kSplitLanes < splits; k += 4 * kSplitLanes) {
+            double v[4];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = parts[(int64_t)(k + u) * stride + i];
+            for (int u = 0; u < 4; ++u) v[u] = parts[(int64_t)(k + u * kSplitLanes) * stride + i];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
+            for (int u = 0; u < 4; ++u) s += v[u];
+        }
+        for (; k < splits; k += kSplitLanes) s += parts[(int64_t)k * stride + i];
     }
-    for (; k < splits; ++k) s += parts[(int64_t)k * stride + i];
-    y[i] = (mode == 1) ? b[i] - s : s;
+#pragma unroll
+    for (int off = 1; off < kSplitLanes; off <<= 1) s += __shfl_xor(s, off, 64);
+    if (q == 0 && i < n) y[i] = (mode == 1) ? b[i] - s : s;
 }
 
 constexpr int kDotThreads = 256;
@@ -197,15 +206,27 @@ __global__ void scale_copy_kernel(const double* __restrict__ w, const double* __
     v[i] = w[i] * inv;
 }
 
-// xc = x + V[:, :k] y
-__global__ void vy_plus_x_kernel(const double* __restrict__ V, int64_t ldv, int k, const double* __restrict__ y,
-                                 const double* __restrict__ x, int n, double* __restrict__ xc)
+// xc = x + V[:, :k] y with y passed by value (the host's least-squares
+// solution: a host-to-device copy before the launch would stall the queue)
+constexpr int kYArg = 64;
+struct YArg {
+    double v[kYArg];
+};
+
+__global__ void vy_plus_x_kernel(const double* __restrict__ V, int64_t ldv, int k, YArg y, const double* x, int n,
+                                 double* xc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int j = 0; j < k; ++j) s += V[(int64_t)j * ldv + i] * y[j];
+    for (int j = 0; j < k; ++j) s += V[(int64_t)j * ldv + i] * y.v[j];
     xc[i] = x ? x[i] + s : s;
+}
+
+__global__ void copy_vec_kernel(const double* __restrict__ src, double* __restrict__ dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 __global__ void copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, size_t n4)
@@ -220,7 +241,23 @@ __global__ void f64_to_f32_kernel(const double* __restrict__ src, float* __restr
         dst[i] = (float)src[i];
 }
 
+// A few words into coherent pinned host memory, by one workgroup: the
+// runtime's device-to-host copy ends with a system-scope release that stalls
+// the queue ~15 us, once per GMRES iteration.
+__global__ void to_host_kernel(const unsigned* __restrict__ src, unsigned* dst, int words)
+{
+    for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t launch_to_host(const void* src, void* host_dst, size_t bytes, hipStream_t s)
+{
+    if (bytes % sizeof(unsigned) != 0 || bytes > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(to_host_kernel, dim3(1), dim3(256), 0, s, static_cast<const unsigned*>(src),
+                       static_cast<unsigned*>(host_dst), (int)(bytes / sizeof(unsigned)));
+    return hipGetLastError();
+}
 
 hipError_t launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s)
 {
@@ -285,15 +322,15 @@ hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double
         e = hipEventRecord(ev1, s);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(reduce_splits_kernel, dim3((p.rows + 255) / 256), dim3(256), 0, s, parts, p.splits, pstride,
-                       p.rows, b, mode, y);
+    hipLaunchKernelGGL(reduce_splits_kernel, dim3((p.rows * kSplitLanes + 255) / 256), dim3(256), 0, s, parts,
+                       p.splits, pstride, p.rows, b, mode, y);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_splits(const double* parts, int splits, int64_t stride, int n, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(reduce_splits_kernel, dim3((n + 255) / 256), dim3(256), 0, s, parts, splits, stride, n,
-                       nullptr, 0, out);
+    hipLaunchKernelGGL(reduce_splits_kernel, dim3((n * kSplitLanes + 255) / 256), dim3(256), 0, s, parts, splits,
+                       stride, n, nullptr, 0, out);
     return hipGetLastError();
 }
 
@@ -325,10 +362,25 @@ hipError_t launch_scale_copy(const double* w, const double* nrm2, int idx, int n
     return hipGetLastError();
 }
 
-hipError_t launch_vy_plus_x(const double* V, int64_t ldv, int k, const double* y, const double* x, int n,
+hipError_t launch_vy_plus_x(const double* V, int64_t ldv, int k, const double* y_host, const double* x, int n,
                             double* xc, hipStream_t s)
 {
-    hipLaunchKernelGGL(vy_plus_x_kernel, dim3((n + 255) / 256), dim3(256), 0, s, V, ldv, k, y, x, n, xc);
+    for (int c = 0; c < k || c == 0; c += kYArg) {  // chunks of 64 basis vectors
+        YArg y{};
+        const int kc = std::min(kYArg, k - c);
+        for (int j = 0; j < kc; ++j) y.v[j] = y_host[c + j];
+        hipLaunchKernelGGL(vy_plus_x_kernel, dim3((n + 255) / 256), dim3(256), 0, s, V + (int64_t)c * ldv, ldv,
+                           std::max(0, kc), y, c == 0 ? x : xc, n, xc);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if (kc <= 0) break;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_copy_vec(const double* src, double* dst, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(copy_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, n);
     return hipGetLastError();
 }
 

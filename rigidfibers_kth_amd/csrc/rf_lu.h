@@ -26,12 +26,8 @@ namespace rf {
 size_t lu_workspace_bytes(int n, int update_fp16);
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
                      int block_pivot, int update_fp16, void* ws, int* redone, hipStream_t s);
-// x <- U^-1 L^-1 P x with FP64 vectors (substitution, two launches per
-// 64-row block).
-hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s);
-
-// The same solve with one launch per 64-row block and sweep, for the factors
-// that precondition every GMRES iteration.  aux holds
+// x <- U^-1 L^-1 P x with FP64 vectors, one launch per 64-row block and
+// sweep (FP32 or FP64 factors).  aux holds
 // lu_solve_aux_bytes(n, elem_bytes): the row permutation and the inverses of
 // the 64x64 diagonal blocks of L and U, filled by lu_prepare_solve once per
 // factorisation; tmp is n doubles.

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace rf {
@@ -264,7 +265,9 @@ __device__ __forceinline__ void load_block(const T* A, int64_t lda, int k0, int 
 // owners of column j+1 also find their partial argmax for the next column.
 // The update loads 8 rows before it stores any (LDS latency overlapped).
 constexpr int kDiagThreads = 1024;
-constexpr int kSuperPanels = 8;  // panels per trailing update
+// columns per trailing update (super-panel): 1024 for FP32 factors, 512 for FP64
+template <typename T>
+constexpr int super_width() { return sizeof(T) == 4 ? 1024 : 512; }
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(kDiagThreads) lu_diag_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv,
@@ -382,6 +385,103 @@ __global__ void __launch_bounds__(kDiagThreads) lu_diag_kernel(T* A, int64_t lda
     }
     if (t < nb) ipiv[k0 + t] = k0 + piv_l[t];
     if (t == 0 && zero_l) info[1] = 1;
+}
+
+// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N-1>{})
+template <typename F, int... J>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, J...>)
+{
+    (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Value of lane `src` (wave-uniform) of a wavefront.
+__device__ __forceinline__ float lane_value(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_value(double v, int src)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Diagonal block of width nb <= 64 by one wavefront: lane r holds row r (64
+// values in registers, the column loop fully unrolled), so a column step
+// needs no barrier and no LDS.  Rows are not moved while the block is
+// factorised: each lane keeps the position its row holds in LAPACK's
+// swapped order (the pivot search breaks ties by that position, as idamax
+// does), the pivot row's entries are broadcast with readlane, and every row
+// is stored at its final position at the end.  bak receives the original
+// values, ipiv and info[1] as in lu_diag_kernel.
+template <typename T>
+__global__ void __launch_bounds__(64) lu_diag64_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv, int* info,
+                                                       T* __restrict__ bak, int64_t ldb)
+{
+    const int r = threadIdx.x;
+    const bool live = r < nb;
+    T a[64];
+#pragma unroll
+    for (int c = 0; c < 64; ++c) a[c] = (live && c < nb) ? A[(int64_t)(k0 + c) * lda + k0 + r] : T(0);
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+        if (live && c < nb) bak[(int64_t)c * ldb + r] = a[c];
+    int pos = r, my_ipiv = r;
+    bool done = !live;
+    bool zero = false;
+    // compile-time column indices keep a[] in registers (a 64-step loop is too
+    // large for the unroller)
+    static_for<64>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if (j < nb) {  // uniform
+            // first largest |a_rj| over the rows not yet pivots, ties to the smaller position
+            double bv = done ? -1.0 : fabs((double)a[j]);
+            int bk = done ? 1 << 20 : pos, bl = r;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(bv, off, 64);
+                const int ok = __shfl_xor(bk, off, 64), ol = __shfl_xor(bl, off, 64);
+                if (ov > bv || (ov == bv && ok < bk)) {
+                    bv = ov;
+                    bk = ok;
+                    bl = ol;
+                }
+            }
+            const int p = __builtin_amdgcn_readfirstlane(bl);
+            const int ppos = __builtin_amdgcn_readfirstlane(bk);
+            if (!(bv > 0.0)) zero = true;
+            if (r == j) my_ipiv = ppos;
+            if (r != p && pos == j) pos = ppos;  // the row at position j moves to the pivot's position
+            const T piv = lane_value(a[j], p);
+            const T inv = (piv != T(0)) ? T(1) / piv : T(0);
+            if (r == p) {
+                pos = j;
+                done = true;
+            }
+            const T l = a[j] * inv;
+            if (!done) a[j] = l;
+            static_for<64>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                if constexpr (c > j) {
+                    const T u = lane_value(a[c], p);
+                    if (!done) a[c] -= l * u;
+                }
+            });
+        }
+    });
+    if (live) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c)
+            if (c < nb) A[(int64_t)(k0 + c) * lda + k0 + pos] = a[c];
+        ipiv[k0 + r] = k0 + my_ipiv;
+    }
+    if (r == 0 && zero) info[1] = 1;  // zero pivot: left to the column-by-column panel
 }
 
 // Broadcast of lane SRC of each group of 4 lanes (DPP quad_perm).
@@ -1006,7 +1106,7 @@ __global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda,
 // and Up[j][k] = U[K0 + k][K1 + j] s_{K1+j} as fp16 with K contiguous (row
 // stride kPackK, zero for K1 - K0 <= k < the width rounded to a chunk), so the
 // update's staging is one 16-byte copy per row segment.
-constexpr int kPackK = kSuperPanels * kMaxPanel;
+constexpr int kPackK = super_width<float>();
 
 // L: a 64-row x 64-K tile per workgroup, transposed through LDS.
 __global__ void __launch_bounds__(256) pack_l_kernel(const float* __restrict__ A, int64_t lda, int n, int K0, int K1,
@@ -1118,60 +1218,6 @@ __global__ void __launch_bounds__(256) lu_gemm_f16p_kernel(float* A, int64_t lda
     f16_update_tile(A, lda, K1 + st.ri, K1 + st.cj, n, n, vec, kwp / kHK, st, colscale, As, Bs);
 }
 
-// ---- solves with the factors (FP64 vectors) ------------------------------
-
-__global__ void apply_pivots_kernel(double* b, int n, const int* __restrict__ ipiv)
-{
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int k = 0; k < n; ++k) {
-        const int p = ipiv[k];
-        if (p != k) {
-            const double t = b[k];
-            b[k] = b[p];
-            b[p] = t;
-        }
-    }
-}
-
-// Solve the diagonal block [k0,k1) of L (unit lower, lower=1) or U
-// (upper, lower=0) for the vector segment in place.  One workgroup of 64.
-template <typename T>
-__global__ void trsv_diag_kernel(const T* A, int64_t lda, double* x, int k0, int k1, int lower)
-{
-    __shared__ double xs[64];
-    const int nb = k1 - k0;
-    const int t = threadIdx.x;
-    if (t < nb) xs[t] = x[k0 + t];
-    __syncthreads();
-    if (lower) {
-        for (int r = 0; r < nb; ++r) {
-            const double xr = xs[r];
-            if (t > r && t < nb) xs[t] -= (double)A[(int64_t)(k0 + r) * lda + k0 + t] * xr;
-            __syncthreads();
-        }
-    } else {
-        for (int r = nb - 1; r >= 0; --r) {
-            if (t == r) xs[r] = xs[r] / (double)A[(int64_t)(k0 + r) * lda + k0 + r];
-            __syncthreads();
-            const double xr = xs[r];
-            if (t < r) xs[t] -= (double)A[(int64_t)(k0 + r) * lda + k0 + t] * xr;
-            __syncthreads();
-        }
-    }
-    if (t < nb) x[k0 + t] = xs[t];
-}
-
-// x[i] -= sum_{j in [k0,k1)} A[i, j] x[j] for rows i in [r0, r1)
-template <typename T>
-__global__ void trsv_update_kernel(const T* A, int64_t lda, double* x, int k0, int k1, int r0, int r1)
-{
-    const int i = r0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= r1) return;
-    double s = 0.0;
-    for (int j = k0; j < k1; ++j) s += (double)A[(int64_t)j * lda + i] * x[j];
-    x[i] -= s;
-}
-
 // Block-pivoted panel [k0, k1): diagonal block, then L21 below it; bak
 // receives the panel's original columns (rows [k0, n), leading dimension n).
 template <typename T>
@@ -1189,9 +1235,8 @@ void block_panel(T* A, int64_t lda, int n, int k0, int k1, int* ipiv, int* info,
             return;
         }
     }
-    // FP64 panels are at most 64 wide
-    hipLaunchKernelGGL((lu_diag_kernel<T, 64>), dim3(1), dim3(kDiagThreads), 0, s, A, lda, k0, nb, ipiv, info, bak,
-                       (int64_t)n);
+    // panels of at most 64 columns (all FP64 ones): one wavefront
+    hipLaunchKernelGGL(lu_diag64_kernel<T>, dim3(1), dim3(64), 0, s, A, lda, k0, nb, ipiv, info, bak, (int64_t)n);
     if (k1 < n)
         hipLaunchKernelGGL((lu_l21_kernel<T, 64>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info, bak, (int64_t)n);
 }
@@ -1233,7 +1278,7 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
     // catch up with the earlier panel (narrow updates), and the trailing
     // matrix takes one update with K = 2 nb per super-panel, which halves its
     // read-modify-write traffic.
-    const int sb = kSuperPanels * nb;
+    const int sb = std::max(1, super_width<T>() / nb) * nb;
     for (int K0 = 0; K0 < n; K0 += sb) {
         const int K1 = std::min(n, K0 + sb);
         for (int k0 = K0; k0 < K1; k0 += nb) {
@@ -1308,29 +1353,6 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
     return hipSuccess;
 }
 
-template <typename T>
-hipError_t solve_impl(const T* LU, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s)
-{
-    constexpr int B = 64;
-    hipLaunchKernelGGL(apply_pivots_kernel, dim3(1), dim3(64), 0, s, x, n, ipiv);
-    for (int k0 = 0; k0 < n; k0 += B) {
-        const int k1 = std::min(n, k0 + B);
-        hipLaunchKernelGGL(trsv_diag_kernel<T>, dim3(1), dim3(64), 0, s, LU, lda, x, k0, k1, 1);
-        if (k1 < n)
-            hipLaunchKernelGGL(trsv_update_kernel<T>, dim3((n - k1 + 255) / 256), dim3(256), 0, s, LU, lda, x, k0, k1,
-                               k1, n);
-    }
-    const int last0 = ((n - 1) / B) * B;
-    for (int k0 = last0; k0 >= 0; k0 -= B) {
-        const int k1 = std::min(n, k0 + B);
-        hipLaunchKernelGGL(trsv_diag_kernel<T>, dim3(1), dim3(64), 0, s, LU, lda, x, k0, k1, 0);
-        if (k0 > 0)
-            hipLaunchKernelGGL(trsv_update_kernel<T>, dim3((k0 + 255) / 256), dim3(256), 0, s, LU, lda, x, k0, k1, 0,
-                               k0);
-    }
-    return hipGetLastError();
-}
-
 // One wavefront: operands encode (mode 0) A[i][0] = i+1, B[0][j] = 1 -> D = i+1;
 // (mode 1) A[i][0] = 1, B[0][j] = j+1 -> D = j+1; (mode 2) only the lanes
 // 32..47 of A and B are 1 -> D = 1 where both use the same K slot.
@@ -1360,9 +1382,9 @@ __global__ void mfma_probe_kernel(int mode, double* out64, float* out32)
     }
 }
 
-// ---- solves with FP32 factors: permutation + one launch per block ----------
-// A solve with the factors of the mixed mode runs once per GMRES iteration,
-// so it is built for few launches and no serial chains: once per
+// ---- solves with the factors: permutation + one launch per block -----------
+// A solve with preconditioner factors runs once per GMRES iteration, so it
+// is built for few launches and no serial chains: once per
 // factorisation the pivots become one permutation and the 64x64 diagonal
 // blocks of L and U are inverted (one workgroup per block); each 64-row
 // block of the forward and backward sweeps is then one launch, in which
@@ -1557,12 +1579,6 @@ hipError_t lu_solve_blocks(const void* LU, int elem_bytes, int64_t lda, int n, c
     if (elem_bytes == 8)
         return solve_blocks_impl<double>(static_cast<const double*>(LU), lda, n, aux, x, tmp, s);
     return solve_blocks_impl<float>(static_cast<const float*>(LU), lda, n, aux, x, tmp, s);
-}
-
-hipError_t lu_solve(const void* LU, int elem_bytes, int64_t lda, int n, const int* ipiv, double* x, hipStream_t s)
-{
-    if (elem_bytes == 8) return solve_impl<double>(static_cast<const double*>(LU), lda, n, ipiv, x, s);
-    return solve_impl<float>(static_cast<const float*>(LU), lda, n, ipiv, x, s);
 }
 
 }  // namespace rf

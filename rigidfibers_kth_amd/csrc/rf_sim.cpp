@@ -173,7 +173,6 @@ int rf_sim::alloc_all()
     ALLOC(d_gparts, std::max(gemv_parts_elems(planA), gemv_parts_elems(planW)));
     ALLOC(d_dparts, (size_t)multidot_blocks(n) * (m + 2) + 64);
     ALLOC(d_dots, (size_t)(3 * (m + 2) + 8));
-    ALLOC(d_y, (size_t)(m + 2));
     ALLOC(d_counter, 1);
 #undef ALLOC
     HIP_TRY(hipMemsetAsync(d_counter, 0, sizeof(unsigned int), stream));
@@ -264,7 +263,7 @@ int rf_sim::gemv_A(const double* x, double* y, const double* b, int mode)
 
 int rf_sim::fetch_dots(int count)
 {
-    HIP_TRY(hipMemcpyAsync(h_dots, d_dots, sizeof(double) * count, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(launch_to_host(d_dots, h_dots, sizeof(double) * count, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     resolve_timed();
     return RF_OK;
@@ -317,10 +316,10 @@ static void rot1(double* x, double* y, double c, double s)
 int rf_sim::update_iterate(int k)
 {
     if (!prec_ready) {
-        HIP_TRY(launch_vy_plus_x(d_V, lda, k, d_y, d_x, n, d_xc, stream));
+        HIP_TRY(launch_vy_plus_x(d_V, lda, k, h_y, d_x, n, d_xc, stream));
         return RF_OK;
     }
-    HIP_TRY(launch_vy_plus_x(d_V, lda, k, d_y, nullptr, n, d_t, stream));
+    HIP_TRY(launch_vy_plus_x(d_V, lda, k, h_y, nullptr, n, d_t, stream));
     int rc = precond_apply(d_t, d_z);
     if (rc) return rc;
     HIP_TRY(launch_axpby(n, 1.0, d_x, 1.0, d_z, d_xc, stream));
@@ -404,7 +403,7 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
             // The Hessenberg column goes to the host; the next basis vector is
             // already on the device, so its product is queued before the host
             // waits (it is discarded if this iteration ends the solve).
-            HIP_TRY(hipMemcpyAsync(h_dots, d_dots, sizeof(double) * 3 * (m + 2), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(launch_to_host(d_dots, h_dots, sizeof(double) * 3 * (m + 2), stream));
             HIP_TRY(hipEventRecord(ev_k0, stream));
             if (k < m && total_before(iterOut, m, k) < maxit) {
                 if ((rc = matvec(k))) return rc;
@@ -425,7 +424,6 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
             if (bea <= tol || total >= maxit) {
                 hess_solve(H, ldh, g, k, y);
                 std::copy(y.begin(), y.begin() + k, h_y);
-                HIP_TRY(hipMemcpyAsync(d_y, h_y, sizeof(double) * k, hipMemcpyHostToDevice, stream));
                 if ((rc = update_iterate(k))) return rc;
                 have_xc = true;
                 if ((rc = gemv_A(d_xc, d_r, d_b, 1))) return rc;
@@ -433,7 +431,7 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
                 if ((rc = norm2(d_r, &rr))) return rc;
                 const double be = std::sqrt(rr) / bn;
                 if (be <= tol || total >= maxit) {
-                    HIP_TRY(hipMemcpyAsync(d_x, d_xc, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+                    HIP_TRY(launch_copy_vec(d_xc, d_x, n, stream));
                     *iters_out = total;
                     *be_out = be;
                     *status_out = (be <= tol) ? 0 : -4;
@@ -445,10 +443,9 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
         if (bea > tol || !have_xc) {
             hess_solve(H, ldh, g, m, y);
             std::copy(y.begin(), y.begin() + m, h_y);
-            HIP_TRY(hipMemcpyAsync(d_y, h_y, sizeof(double) * m, hipMemcpyHostToDevice, stream));
             if ((rc = update_iterate(m))) return rc;
         }
-        HIP_TRY(hipMemcpyAsync(d_x, d_xc, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(launch_copy_vec(d_xc, d_x, n, stream));
         if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
         if ((rc = norm2(d_r, &beta2))) return rc;
         beta = std::sqrt(beta2);
@@ -491,9 +488,10 @@ int rf_sim::factorize(bool* done, bool precond_only)
         d_lu_ws = static_cast<char*>(q) + piv_bytes;
         HIP_TRY(hipMemsetAsync(d_lu_ws, 0, lu_workspace_bytes(n, ws_fp16), stream));
     }
-    // panel width: 128 for FP32 factors (halves the trailing update's traffic), 64 for FP64
+    // panel width 64 (one wavefront factorises the diagonal block); FP32
+    // factors accept up to 128
     const int nb_max = (es == 4) ? 128 : 64;
-    const int nb = (p.lu_block > 0) ? std::min(nb_max, p.lu_block) : nb_max;
+    const int nb = (p.lu_block > 0) ? std::min(nb_max, p.lu_block) : 64;
     hipEvent_t ea = nullptr, eb = nullptr;
     int rc0;
     if (p.profile) {
@@ -525,7 +523,7 @@ int rf_sim::factorize(bool* done, bool precond_only)
         rf::set_error("LU: matrix is singular (U(" + std::to_string(info[0]) + "," + std::to_string(info[0]) + ") = 0)");
         return RF_ENOCONV;
     }
-    if (es == 4) {  // FP32 factors are applied every iteration: permutation for the block sweeps
+    {  // permutation and diagonal-block inverses for the block sweeps
         if (!d_lu_aux) {
             void* q = nullptr;
             if (hipMalloc(&q, lu_solve_aux_bytes(n, es) + sizeof(double) * (size_t)lda) != hipSuccess) {
@@ -546,10 +544,7 @@ int rf_sim::factorize(bool* done, bool precond_only)
 // x <- (LU)^-1 x with the current factors
 int rf_sim::lu_apply(double* x)
 {
-    if (lu_es == 4)
-        HIP_TRY(lu_solve_blocks(d_LU, lu_es, lda, n, d_lu_aux, x, d_lu_tmp, stream));
-    else
-        HIP_TRY(lu_solve(d_LU, lu_es, lda, n, d_ipiv, x, stream));
+    HIP_TRY(lu_solve_blocks(d_LU, lu_es, lda, n, d_lu_aux, x, d_lu_tmp, stream));
     return RF_OK;
 }
 
@@ -569,7 +564,7 @@ int rf_sim::solve_lu(double* be_out)
     if ((rc = norm2(d_b, &bn2))) return rc;
     const double bn = std::sqrt(bn2);
     if (es == 8) {
-        HIP_TRY(hipMemcpyAsync(d_x, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(launch_copy_vec(d_b, d_x, n, stream));
         if ((rc = lu_apply(d_x))) return rc;
         if ((rc = gemv_A(d_x, d_r, d_b, 1))) return rc;
         double rr;
@@ -580,7 +575,7 @@ int rf_sim::solve_lu(double* be_out)
     // iterative refinement: x += LU^-1 (b - A x)
     const double tol = (p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : p.gmres_tol;
     HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, stream));
-    HIP_TRY(hipMemcpyAsync(d_r, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(launch_copy_vec(d_b, d_r, n, stream));
     double be = 1.0;
     for (int it = 0; it < 30; ++it) {
         if ((rc = lu_apply(d_r))) return rc;
@@ -683,7 +678,7 @@ int rf_sim::learned_guess()
     }
     if ((rc = gemv_A(d_xprev, u, nullptr, 0))) return rc;
     if ((rc = gemv_A(d_guess, v, nullptr, 0))) return rc;
-    HIP_TRY(hipMemcpyAsync(bb, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(launch_copy_vec(d_b, bb, n, stream));
     // D[0..2] = (u.b, v.b, b.b), D[3..4] = (u.v, v.v), D[5] = u.u
     HIP_TRY(launch_multidot(d_V, ldv, 2, bb, n, d_dparts, d_counter, d_dots, stream));
     HIP_TRY(launch_multidot(d_V, ldv, 1, v, n, d_dparts, d_counter, d_dots + 3, stream));
@@ -772,7 +767,7 @@ int rf_sim::setup_precond(int kind)
 int rf_sim::precond_apply(const double* r, double* z)
 {
     if (prec_kind == RF_PRECOND_LU) {
-        if (r != z) HIP_TRY(hipMemcpyAsync(z, r, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        if (r != z) HIP_TRY(launch_copy_vec(r, z, n, stream));
         return lu_apply(z);
     }
     HIP_TRY(launch_ras_apply(N, ras_k, knn, 3 * M, d_nbr, d_B, r, z, stream));
@@ -844,10 +839,10 @@ int rf_sim::solve()
         HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, stream));
     } else if (p.warm_start == RF_WARM_EXTRAP && solves_done >= 2) {
         HIP_TRY(launch_axpby(n, 2.0, d_x, -1.0, d_xprev, d_guess, stream));
-        HIP_TRY(hipMemcpyAsync(d_xprev, d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
-        HIP_TRY(hipMemcpyAsync(d_x, d_guess, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(launch_copy_vec(d_x, d_xprev, n, stream));
+        HIP_TRY(launch_copy_vec(d_guess, d_x, n, stream));
     } else {
-        HIP_TRY(hipMemcpyAsync(d_xprev, d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(launch_copy_vec(d_x, d_xprev, n, stream));
         if (p.warm_start == RF_WARM_LEARNED && mlp_weights && !p.use_direct_solver && (rc = learned_guess()))
             return rc;
     }
@@ -1039,7 +1034,10 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     const int es = s->mixed ? 4 : 8;
     s->planA = plan_gemv(s->n, s->n, s->lda, es, s->num_cus);
     s->planW = plan_gemv(6 * s->N, s->wcols, s->ldw, es, s->num_cus);
-    if (hipHostMalloc((void**)&s->h_dots, sizeof(double) * (3 * (s->restart + 2) + 8)) != hipSuccess ||
+    // coherent (fine-grained) pinned memory: the GPU's writes into it bypass
+    // its caches, so the per-iteration event needs no system-scope fence
+    if (hipHostMalloc((void**)&s->h_dots, sizeof(double) * (3 * (s->restart + 2) + 8), hipHostMallocCoherent) !=
+            hipSuccess ||
         hipHostMalloc((void**)&s->h_y, sizeof(double) * (s->restart + 2)) != hipSuccess) {
         delete s;
         set_error("rf_create: hipHostMalloc failed");
@@ -1051,7 +1049,10 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
         return RF_EHIP;
     }
     for (hipEvent_t* ev : {&s->ev_p0, &s->ev_p1, &s->ev_k0, &s->ev_k1}) {
-        if (hipEventCreate(ev) != hipSuccess) {
+        // ev_k0 marks the Hessenberg column's copy into h_dots once per GMRES
+        // iteration; a system-scope release there stalls the queue ~15 us
+        const unsigned flags = (ev == &s->ev_k0) ? (hipEventDisableTiming | hipEventDisableSystemFence) : 0u;
+        if (hipEventCreateWithFlags(ev, flags) != hipSuccess) {
             delete s;
             set_error("rf_create: hipEventCreate failed");
             return RF_EHIP;
@@ -1504,7 +1505,7 @@ extern "C" int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t
         set_error("rf_selftest_lu: bad arguments");
         return RF_EINVAL;
     }
-    if (nb == 0) nb = nb_max;
+    if (nb == 0) nb = 64;
     const int64_t lda = (n + 3) / 4 * 4;
     const size_t es = (size_t)elem_bytes;
     std::vector<char> host((size_t)lda * n * es, 0);

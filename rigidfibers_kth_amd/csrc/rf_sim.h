@@ -56,7 +56,6 @@ struct rf_sim {
     double* d_gparts = nullptr;
     double* d_dparts = nullptr;
     double* d_dots = nullptr;
-    double* d_y = nullptr;
     unsigned int* d_counter = nullptr;
     double* h_dots = nullptr;  // pinned
     double* h_y = nullptr;     // pinned

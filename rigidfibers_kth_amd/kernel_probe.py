@@ -2,7 +2,7 @@
 
     python3 -m rigidfibers_kth_amd.kernel_probe gemv --fibers 1000 --reps 20
     python3 -m rigidfibers_kth_amd.kernel_probe copy --bytes 1073741824 --reps 5
-    python3 -m rigidfibers_kth_amd.kernel_probe lu --fibers 1000 [--fp32 [--fp16]] [--column-panel]
+    python3 -m rigidfibers_kth_amd.kernel_probe lu --fibers 1000 [--fp32 [--fp16]] [--column-panel] [--block NB]
     python3 -m rigidfibers_kth_amd.kernel_probe assemble --fibers 1000 --reps 5
 
 The layouts are the seeded ones of the benchmark (volume fraction 1e-2 for
@@ -40,6 +40,7 @@ def main(argv=None):
     ap.add_argument("--fp16", action="store_true", help="lu: FP32 factors with fp16-operand trailing updates, "
                     "factorised as the preconditioner of one GMRES solve")
     ap.add_argument("--column-panel", action="store_true", help="lu: pivot search over whole columns in every panel")
+    ap.add_argument("--block", type=int, default=0, help="lu: panel width (0 = default)")
     a = ap.parse_args(argv)
     if a.kernel == "copy":
         ms = selftest_copy(a.bytes, a.reps)
@@ -49,7 +50,7 @@ def main(argv=None):
     prec = PREC_MIXED if (a.fp32 and a.kernel != "lu") else PREC_FP64
     kw = dict(precision=prec, profile=1)
     if a.kernel == "lu":
-        kw.update(lu_factor_fp32=int(a.fp32 or a.fp16), lu_column_panel=int(a.column_panel))
+        kw.update(lu_factor_fp32=int(a.fp32 or a.fp16), lu_column_panel=int(a.column_panel), lu_block=a.block)
         if a.fp16:
             kw.update(precond=PRECOND_LU, lu_update_fp16=1, gmres_restart=30, gmres_max_iters=300, gmres_tol=1e-10)
         else:

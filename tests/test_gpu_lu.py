@@ -62,7 +62,7 @@ def test_block_panel_redoes_panels_that_pivot_outside_the_block(n, nb, es):
     if es == 4:
         A = A.astype(np.float32).astype(np.float64)
     LU, ipiv, info, fb = rf.selftest_lu(A, es, nb, column_panel=0)
-    width = nb if nb else (128 if es == 4 else 64)
+    width = nb if nb else 64
     assert info == 0 and fb == (n - 1) // width
     lu_ref, piv_ref = sl.lu_factor(A)
     if es == 8:
@@ -76,7 +76,7 @@ def test_block_panel_redoes_panels_that_pivot_outside_the_block(n, nb, es):
 @pytest.mark.parametrize("n,nb,es", [(300, 0, 8), (301, 40, 8), (250, 13, 8), (515, 0, 4), (300, 100, 4),
                                      (200, 64, 4)])
 def test_block_panel_equals_partial_pivoting_when_pivots_stay_in_the_block(n, nb, es):
-    width = nb if nb else (128 if es == 4 else 64)
+    width = nb if nb else 64
     A = _dominant_with_block_swaps(n, width, 3 + n)
     if es == 4:
         A = A.astype(np.float32).astype(np.float64)
@@ -118,7 +118,7 @@ def test_fp16_update_factors(n, nb):
     partial pivoting and a backward error of the order of fp16 rounding;
     scaled copies of the matrix (2^-30, 2^30) give the same relative error
     (per-column scaling of U12 keeps fp16 away from underflow and overflow)."""
-    A = _dominant_with_block_swaps(n, nb if nb else 128, 17 + n)
+    A = _dominant_with_block_swaps(n, nb if nb else 64, 17 + n)
     A = A.astype(np.float32).astype(np.float64)
     _, piv_ref = sl.lu_factor(A)
     errs = []

@@ -6,14 +6,15 @@ Headline workload (`value`): N=1000 rigid fibers of length 2, slenderness
 uniform on the sphere (seeded, overlaps rejected), gravity (0,0,-1); M=5
 Legendre terms, 8x3 Gauss points, analytic inner integrals, dt=0.1; solved the
 reference's way in FP64: GMRES(10) to a backward error of 1e-10, warm-started
-from the previous step, LU fallback after 1000 iterations
+from the previous step, LU fallback after 300 iterations
 (fortran/coeffs_and_vels.f90:253-339).
 A "step" is one full timestep (assemble, solve, velocities, advance).
 
 Beside it, on the same initial state: the mixed-precision + learned path
-(FP32 matrix with FP64 near field and closed forms, FP64 residuals and
-Arnoldi, neighbour preconditioner switching to FP32-LU GMRES-IR on hard
-systems, learned initial guess trained on the GPU on solved pairs), its
+(FP64 operator, residuals and Arnoldi; neighbour preconditioner switching to
+GMRES right-preconditioned by FP32 LU factors with fp16-operand trailing
+updates on hard systems; learned initial guess trained on the GPU on solved
+pairs), its
 relative position error against the plain run after the same steps, the FP64
 direct solver, and the same comparison at N=4000, volume fraction 5e-2, where
 the plain method is timed both as GMRES with LU fallback and as the FP64
@@ -53,10 +54,12 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemv_f64_n15000_traffic.json")
 C2 = dict(N=1000, phi=1e-2, seed=0x5EED0001)
 C4 = dict(N=4000, phi=5e-2, seed=0x5EED0003)
 # the reference's method: FP64 GMRES(10), previous-step guess, LU fallback
-# (fortran/coeffs_and_vels.f90:331-335) after 1000 iterations: beyond about
-# the cost of one FP64 LU in matvecs (~650 at N=1000, ~1500 at N=4000) more
-# GMRES iterations cannot pay off
-PLAIN = dict(gmres_restart=10, gmres_max_iters=1000, gmres_tol=1e-10)
+# (fortran/coeffs_and_vels.f90:331-335) after 300 iterations: about the cost
+# of one FP64 LU and solve at N=1000 (112 ms + 5 ms = 300 iterations of
+# 0.38 ms); beyond it more GMRES iterations cannot pay off.  Measured at
+# N=1000: cap 200 12.75, 300 12.86, 500 10.84, 1000 8.66 steps/s.  At N=4000
+# every step needs more and falls back (the direct solver is timed beside it)
+PLAIN = dict(gmres_restart=10, gmres_max_iters=300, gmres_tol=1e-10)
 # mixed precision + learned: FP64 operator, residuals and Arnoldi; FP32 LU
 # factors (trailing updates on fp16-rounded operands, FP32 accumulation, on
 # the matrix cores) as the preconditioner of hard systems (GMRES-IR) after
@@ -319,7 +322,7 @@ def main():
         "data": "synthetic: seeded fiber layouts (N=1000 seed 0x5EED0001, N=4000 seed 0x5EED0003), overlaps rejected",
         "config": {
             "workload": "plain FP64, N=1000 fibers, volume fraction 1e-2, M=5, 8x3 Gauss points, analytic, "
-                        "GMRES(10) tol 1e-10 with LU fallback after 1000 iterations, previous-step warm start, dt=0.1",
+                        "GMRES(10) tol 1e-10 with LU fallback after 300 iterations, previous-step warm start, dt=0.1",
             "fibers": C2["N"], "unknowns": n, "volume_fraction": C2["phi"], "solver": "gmres(10)", "tol": 1e-10,
         },
         "roofline": {

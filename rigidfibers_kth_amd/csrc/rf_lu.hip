@@ -412,6 +412,34 @@ __device__ __forceinline__ double lane_value(double v, int src)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// max(v, v of the lane selected by DPP control CTRL) for the rows in ROWMASK;
+// lanes without a source (or outside ROWMASK) keep v.  v >= -1 here.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_fmax(double v)
+{
+    const unsigned long long vb = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long ib = (unsigned long long)__double_as_longlong(-2.0);
+    const unsigned lo =
+        (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)ib, (int)(unsigned)vb, CTRL, ROWMASK, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(ib >> 32), (int)(unsigned)(vb >> 32), CTRL,
+                                                              ROWMASK, 0xF, false);
+    return fmax(v, __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo)));
+}
+
+// Largest v over the wavefront: prefix maxima inside each row of 16 lanes
+// (row_shr 1, 2, 4, 8), then across rows (row_bcast 15 and 31); lane 63 ends
+// with the maximum.
+__device__ __forceinline__ double wave_max(double v)
+{
+    v = dpp_fmax<0x111, 0xF>(v);
+    v = dpp_fmax<0x112, 0xF>(v);
+    v = dpp_fmax<0x114, 0xF>(v);
+    v = dpp_fmax<0x118, 0xF>(v);
+    v = dpp_fmax<0x142, 0xA>(v);
+    v = dpp_fmax<0x143, 0xC>(v);
+    return lane_value(v, 63);
+}
+
 // Diagonal block of width nb <= 64 by one wavefront: lane r holds row r (64
 // values in registers, the column loop fully unrolled), so a column step
 // needs no barrier and no LDS.  Rows are not moved while the block is
@@ -441,21 +469,22 @@ __global__ void __launch_bounds__(64) lu_diag64_kernel(T* A, int64_t lda, int k0
         constexpr int j = decltype(jc)::value;
         if (j < nb) {  // uniform
             // first largest |a_rj| over the rows not yet pivots, ties to the smaller position
-            double bv = done ? -1.0 : fabs((double)a[j]);
-            int bk = done ? 1 << 20 : pos, bl = r;
+            const double v = done ? -1.0 : fabs((double)a[j]);
+            const double vmax = wave_max(v);
+            const unsigned long long cand = __ballot(v == vmax);
+            int p;
+            if (__popcll(cand) == 1) {
+                p = __ffsll((unsigned long long)cand) - 1;
+            } else {  // exact ties (or no candidate left): the smallest position
+                int k = (v == vmax) ? pos : 1 << 20;
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(bv, off, 64);
-                const int ok = __shfl_xor(bk, off, 64), ol = __shfl_xor(bl, off, 64);
-                if (ov > bv || (ov == bv && ok < bk)) {
-                    bv = ov;
-                    bk = ok;
-                    bl = ol;
-                }
+                for (int off = 32; off > 0; off >>= 1) k = min(k, __shfl_xor(k, off, 64));
+                const unsigned long long at = __ballot(v == vmax && pos == k);
+                p = at ? __ffsll((unsigned long long)at) - 1 : 0;
             }
-            const int p = __builtin_amdgcn_readfirstlane(bl);
-            const int ppos = __builtin_amdgcn_readfirstlane(bk);
-            if (!(bv > 0.0)) zero = true;
+            p = __builtin_amdgcn_readfirstlane(p);
+            const int ppos = __builtin_amdgcn_readlane(pos, p);
+            if (!(vmax > 0.0)) zero = true;
             if (r == j) my_ipiv = ppos;
             if (r != p && pos == j) pos = ppos;  // the row at position j moves to the pivot's position
             const T piv = lane_value(a[j], p);

@@ -32,7 +32,7 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kColThreads = 256;
 constexpr int kColMaxBlocks = 512;
-constexpr int kMaxPanel = 128;  // panel width: 128 for FP32 factors, 64 for FP64
+constexpr int kMaxPanel = 128;  // widest panel: 128 for FP32 factors (64 by default), 64 for FP64
 
 // (value, index) maximum with ties to the smaller index (LAPACK idamax keeps
 // the first maximal entry) over one workgroup; result valid in thread 0.

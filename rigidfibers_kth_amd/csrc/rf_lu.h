@@ -22,10 +22,11 @@ namespace rf {
 // column by column (block_pivot = 0: every panel that way), so the pivots
 // are partial pivoting's either way.  *redone (may be null) receives the
 // number of panels redone.  Block pivoting synchronises the stream once per
-// panel.  info[0..1] are zero on entry.
+// panel, reading the check through host_flag (one int of coherent pinned
+// host memory) when it is not null.  info[0..1] are zero on entry.
 size_t lu_workspace_bytes(int n, int update_fp16);
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     int block_pivot, int update_fp16, void* ws, int* redone, hipStream_t s);
+                     int block_pivot, int update_fp16, void* ws, int* redone, int* host_flag, hipStream_t s);
 // x <- U^-1 L^-1 P x with FP64 vectors, one launch per 64-row block and
 // sweep (FP32 or FP64 factors).  aux holds
 // lu_solve_aux_bytes(n, elem_bytes): the row permutation and the inverses of

@@ -1276,7 +1276,7 @@ size_t ws_pack_offset(int n) { return ws_backup_offset(n) + (size_t)n * kMaxPane
 
 template <typename T>
 hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, int block_pivot,
-                       int update_fp16, void* ws, int* redone, hipStream_t s)
+                       int update_fp16, void* ws, int* redone, int* host_flag, hipStream_t s)
 {
     double* pval = static_cast<double*>(ws);
     int* pidx = reinterpret_cast<int*>(pval + kColMaxBlocks);
@@ -1318,8 +1318,15 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
                 block_panel<T>(A, lda, n, k0, k1, ipiv, info, bak, s);
                 // the check decides before anything else reads the panel
                 int flag = 0;
-                hipError_t e = hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s);
-                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                hipError_t e;
+                if (host_flag) {  // own copy kernel: the runtime's device-to-host copy stalls the queue
+                    e = launch_to_host(info + 1, host_flag, sizeof(int), s);
+                    if (e == hipSuccess) e = hipStreamSynchronize(s);
+                    flag = *reinterpret_cast<volatile int*>(host_flag);
+                } else {
+                    e = hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s);
+                    if (e == hipSuccess) e = hipStreamSynchronize(s);
+                }
                 if (e != hipSuccess) return e;
                 if (flag) {
                     hipLaunchKernelGGL(lu_restore_kernel<T>, dim3((n - k0 + 255) / 256, k1 - k0), dim3(256), 0, s, A,
@@ -1533,14 +1540,14 @@ size_t lu_workspace_bytes(int n, int update_fp16)
 }
 
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     int block_pivot, int update_fp16, void* ws, int* redone, hipStream_t s)
+                     int block_pivot, int update_fp16, void* ws, int* redone, int* host_flag, hipStream_t s)
 {
     if (nb < 1 || nb > (elem_bytes == 4 ? kMaxPanel : 64) || !ws || lda % 4 != 0) return hipErrorInvalidValue;
     if (elem_bytes == 8)
         return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, 0, ws,
-                                   redone, s);
+                                   redone, host_flag, s);
     return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, update_fp16, ws,
-                              redone, s);
+                              redone, host_flag, s);
 }
 
 static size_t perm_bytes(int n) { return ((sizeof(int) * (size_t)n + 255) / 256) * 256; }

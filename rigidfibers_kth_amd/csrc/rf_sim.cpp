@@ -508,7 +508,7 @@ int rf_sim::factorize(bool* done, bool precond_only)
     HIP_TRY(hipMemsetAsync(d_info, 0, 2 * sizeof(int), stream));
     int redone = 0;
     HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, p.lu_column_panel ? 0 : 1,
-                      fp16 ? 1 : 0, d_lu_ws, &redone, stream));
+                      fp16 ? 1 : 0, d_lu_ws, &redone, reinterpret_cast<int*>(h_dots), stream));
     stats.lu_panel_fallbacks += redone;
     HIP_TRY(hipMemcpyAsync(h_dots, d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
     if (p.profile) {
@@ -1528,7 +1528,7 @@ extern "C" int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t
     int redone = 0;
     if (e == hipSuccess)
         e = lu_factor(dA, elem_bytes, lda, n, nb, d_ipiv, d_ipiv + n, valu ? 0 : 1, column_panel ? 0 : 1,
-                      update_fp16, dW, &redone, nullptr);
+                      update_fp16, dW, &redone, nullptr, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     std::vector<int> piv((size_t)n + 2);
     if (e == hipSuccess) e = hipMemcpy(piv.data(), dP, sizeof(int) * ((size_t)n + 2), hipMemcpyDeviceToHost);

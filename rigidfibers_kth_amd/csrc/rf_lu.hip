@@ -1606,7 +1606,7 @@ hipError_t solve_blocks_impl(const T* LU, int64_t lda, int n, const void* aux, d
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return hipMemcpyAsync(x, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    return launch_copy_vec(tmp, x, n, s);
 }
 
 hipError_t lu_solve_blocks(const void* LU, int elem_bytes, int64_t lda, int n, const void* aux, double* x,

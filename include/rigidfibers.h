@@ -103,7 +103,11 @@ typedef struct rf_params {
     int32_t lu_update_fp16;       /* 1: FP32 factors (mixed mode or lu_factor_fp32) that precondition GMRES
                                      take trailing updates on fp16-rounded operands (FP32 accumulation);
                                      the direct solver's factors keep FP32 updates */
-    int32_t reserved[1];
+    int32_t lu_reuse_iters;       /* RF_PRECOND_LU / RF_PRECOND_AUTO: > 0 lets the LU factors of an earlier
+                                     step's matrix precondition a solve for up to this many GMRES
+                                     iterations; a solve still above the tolerance then factorises the
+                                     current matrix and continues from the iterate reached (0: only
+                                     factors of the current matrix) */
     double refine_tol;            /* GMRES tolerance once the LU preconditioner is in use (0 = gmres_tol) */
 } rf_params;
 
@@ -134,7 +138,8 @@ typedef struct rf_stats {
     int32_t last_precond;           /* RF_PRECOND_* in use at the end of the last GMRES solve */
     int32_t last_neighbour_iters;   /* iterations of the last GMRES solve under the neighbour preconditioner */
     int64_t lu_panel_fallbacks;     /* LU panels redone with the pivot search over whole columns */
-    double reserved[3];
+    int64_t lu_reused_solves;       /* GMRES solves that converged on the factors of an earlier matrix */
+    double reserved[2];
 } rf_stats;
 
 typedef struct rf_sim rf_sim;
@@ -147,7 +152,8 @@ void rf_params_default(rf_params *p);
  * Lines after the last mandatory one may hold "key value" extensions:
  * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1,
  * lu_block NB, precond none|neighbour|lu|auto, precond_neighbours K, precond_switch_iters I,
- * lu_factor_fp32 0|1, lu_column_panel 0|1, lu_update_fp16 0|1, refine_tol T, mixed_inner_tol T. */
+ * lu_factor_fp32 0|1, lu_column_panel 0|1, lu_update_fp16 0|1, lu_reuse_iters I, refine_tol T,
+ * mixed_inner_tol T. */
 int rf_parse_parameters(const char *path, rf_params *out);
 
 /* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per

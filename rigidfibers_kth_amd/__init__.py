@@ -59,7 +59,7 @@ class rf_params(ctypes.Structure):
         ("lu_factor_fp32", ctypes.c_int32),
         ("lu_column_panel", ctypes.c_int32),
         ("lu_update_fp16", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("lu_reuse_iters", ctypes.c_int32),
         ("refine_tol", ctypes.c_double),
     ]
 
@@ -96,7 +96,8 @@ class rf_stats(ctypes.Structure):
         ("last_precond", ctypes.c_int32),
         ("last_neighbour_iters", ctypes.c_int32),
         ("lu_panel_fallbacks", ctypes.c_int64),
-        ("reserved", ctypes.c_double * 3),
+        ("lu_reused_solves", ctypes.c_int64),
+        ("reserved", ctypes.c_double * 2),
     ]
 
     def as_dict(self):

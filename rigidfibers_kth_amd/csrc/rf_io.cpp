@@ -174,6 +174,8 @@ extern "C" int rf_parse_parameters(const char* path, rf_params* out)
             ok = parse_int(val, &p.lu_column_panel);
         } else if (key == "lu_update_fp16") {
             ok = parse_int(val, &p.lu_update_fp16);
+        } else if (key == "lu_reuse_iters") {
+            ok = parse_int(val, &p.lu_reuse_iters) && p.lu_reuse_iters >= 0;
         } else if (key == "refine_tol") {
             ok = parse_double(val, &p.refine_tol);
         } else if (key == "mixed_inner_tol") {

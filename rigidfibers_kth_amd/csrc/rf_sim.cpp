@@ -506,6 +506,7 @@ int rf_sim::factorize(bool* done, bool precond_only)
     else  // FP32 factors of the FP64 matrix
         HIP_TRY(launch_f64_to_f32(static_cast<const double*>(d_A), static_cast<float*>(d_LU), (size_t)lda * n, stream));
     HIP_TRY(hipMemsetAsync(d_info, 0, 2 * sizeof(int), stream));
+    lu_have = false;
     int redone = 0;
     HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, p.lu_column_panel ? 0 : 1,
                       fp16 ? 1 : 0, d_lu_ws, &redone, reinterpret_cast<int*>(h_dots), stream));
@@ -537,6 +538,7 @@ int rf_sim::factorize(bool* done, bool precond_only)
         HIP_TRY(lu_prepare_solve(d_LU, es, lda, n, d_ipiv, d_lu_aux, stream));
     }
     *done = true;
+    lu_have = true;
     lu_fp16 = fp16;
     return RF_OK;
 }
@@ -764,6 +766,39 @@ int rf_sim::setup_precond(int kind)
     return RF_OK;
 }
 
+// GMRES right-preconditioned by LU factors from the iterate in d_x.  With
+// lu_reuse_iters > 0 the factors of an earlier step's matrix go first (the
+// dynamics change the matrix little per step), for at most that many
+// iterations; a solve still above the tolerance then factorises the current
+// matrix and continues from the iterate reached (d_r already holds its
+// residual).  The stopping test is always on the current matrix.
+int rf_sim::gmres_on_lu(int maxit, int* iters, double* be, int* status)
+{
+    int rc;
+    *iters = 0;
+    *status = 0;
+    if (p.lu_reuse_iters > 0 && lu_have && !lu_factored && d_z) {
+        prec_kind = RF_PRECOND_LU;
+        prec_ready = true;
+        if ((rc = solve_gmres(iters, be, status, std::min(maxit, p.lu_reuse_iters)))) return rc;
+        if (*status == 0) {
+            stats.lu_reused_solves += 1;
+            return RF_OK;
+        }
+        if (*iters >= maxit) return RF_OK;
+        r0_ready = true;
+        const double r_init = stats.last_initial_residual;
+        if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
+        int it2 = 0;
+        if ((rc = solve_gmres(&it2, be, status, maxit - *iters))) return rc;
+        *iters += it2;
+        stats.last_initial_residual = r_init;
+        return RF_OK;
+    }
+    if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
+    return solve_gmres(iters, be, status, maxit);
+}
+
 int rf_sim::precond_apply(const double* r, double* z)
 {
     if (prec_kind == RF_PRECOND_LU) {
@@ -869,8 +904,12 @@ int rf_sim::solve()
             const int budget = std::min(maxit, p.precond_switch_iters > 0 ? p.precond_switch_iters : 2 * restart);
             const double tol = p.gmres_tol;
             const bool lu_first = auto_lu_run > 0 && auto_lu_run % kAutoRetry != 0;
-            if ((rc = setup_precond(lu_first ? RF_PRECOND_LU : RF_PRECOND_NEIGHBOUR))) return rc;
-            if ((rc = solve_gmres(&iters, &be, &status, lu_first ? maxit : std::min(budget, restart)))) return rc;
+            if (lu_first) {
+                if ((rc = gmres_on_lu(maxit, &iters, &be, &status))) return rc;
+            } else {
+                if ((rc = setup_precond(RF_PRECOND_NEIGHBOUR))) return rc;
+                if ((rc = solve_gmres(&iters, &be, &status, std::min(budget, restart)))) return rc;
+            }
             const double r0 = stats.last_initial_residual;
             if (!lu_first && status != 0 && iters < budget && r0 > 0.0 && be > 0.0 && be < r0) {
                 const double rate = std::pow(be / r0, 1.0 / iters);
@@ -883,16 +922,19 @@ int rf_sim::solve()
             }
             stats.last_neighbour_iters = lu_first ? 0 : iters;
             if (!lu_first && status != 0 && iters < maxit) {
-                if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
                 int it2 = 0;
-                if ((rc = solve_gmres(&it2, &be, &status, maxit - iters))) return rc;
+                if ((rc = gmres_on_lu(maxit - iters, &it2, &be, &status))) return rc;
                 iters += it2;
             }
             stats.last_initial_residual = r0;
             auto_lu_run = (prec_kind == RF_PRECOND_LU) ? auto_lu_run + 1 : 0;
         } else {
-            if (p.precond != RF_PRECOND_NONE && (rc = setup_precond(p.precond))) return rc;
-            if ((rc = solve_gmres(&iters, &be, &status, maxit))) return rc;
+            if (p.precond == RF_PRECOND_LU) {
+                if ((rc = gmres_on_lu(maxit, &iters, &be, &status))) return rc;
+            } else {
+                if (p.precond != RF_PRECOND_NONE && (rc = setup_precond(p.precond))) return rc;
+                if ((rc = solve_gmres(&iters, &be, &status, maxit))) return rc;
+            }
             stats.last_neighbour_iters = (p.precond == RF_PRECOND_NEIGHBOUR) ? iters : 0;
         }
         stats.last_precond = prec_kind;

@@ -88,6 +88,7 @@ struct rf_sim {
     void* d_lu_aux = nullptr;  // FP32 factors: permutation and diagonal-block inverses
     double* d_lu_tmp = nullptr;
     bool lu_factored = false;  // d_LU holds the factors of the current matrix
+    bool lu_have = false;      // d_LU holds factors (of the current or an earlier matrix)
     int auto_lu_run = 0;       // RF_PRECOND_AUTO: consecutive solves that ended on the LU
     static constexpr int kAutoRetry = 8;
     bool lu_fp16 = false;      // ... with fp16-rounded trailing updates (preconditioner use only)
@@ -126,6 +127,7 @@ struct rf_sim {
     int mlp_infer(double* x_out);
     int learned_guess();
     int setup_precond(int kind);
+    int gmres_on_lu(int maxit, int* iters, double* be, int* status);
     int precond_apply(const double* r, double* z);
     int update_iterate(int k);
     int phase_begin();

@@ -61,11 +61,11 @@ def test_parse_extension_lines(tmp_path):
     assert p.precision == rf.PREC_MIXED and p.warm_start == rf.WARM_LEARNED and p.profile == 1
     f.write_text(src + "precond auto\nprecond_switch_iters 45\nlu_factor_fp32 1\nrefine_tol 1d-13\n"
                        "precond_neighbours 6\nmixed_inner_tol 1e-11\nlu_block 96\nlu_update_fp16 1\n"
-                       "lu_column_panel 1\n")
+                       "lu_column_panel 1\nlu_reuse_iters 40\n")
     p = rf.Parameters.parse_parameter_file(str(f))
     assert (p.precond, p.precond_switch_iters, p.lu_factor_fp32, p.precond_neighbours, p.lu_block) == \
         (rf.PRECOND_AUTO, 45, 1, 6, 96)
-    assert (p.lu_update_fp16, p.lu_column_panel) == (1, 1)
+    assert (p.lu_update_fp16, p.lu_column_panel, p.lu_reuse_iters) == (1, 1, 40)
     assert p.refine_tol == 1e-13 and p.mixed_inner_tol == 1e-11
     for bad in ("precond sometimes\n", "warm_start later\n", "precision fp16\n"):
         f.write_text(src + bad)

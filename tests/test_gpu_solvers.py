@@ -193,6 +193,38 @@ def test_auto_preconditioner_starts_on_the_lu_after_a_switch():
     assert nbr[0] == 5 and nbr[1:8] == [0] * 7 and nbr[8] == 5 and nbr[9] == 0, nbr
 
 
+@pytest.mark.parametrize("precond", [rf.PRECOND_LU, rf.PRECOND_AUTO])
+def test_lu_reuse_across_steps(precond):
+    """lu_reuse_iters: the factors of an earlier step's matrix precondition
+    the next solves; a solve that does not converge within the limit
+    factorises the current matrix and continues.  Every solve still meets
+    the tolerance on its own matrix, so the trajectory matches the one with
+    a factorisation per step.  A limit of one iteration cannot converge from
+    the previous step's solution and takes the refactorisation branch."""
+    pos, ori = _hard_state()
+    kw = dict(precond=precond, precond_switch_iters=5, lu_factor_fp32=1, lu_update_fp16=1, gmres_tol=1e-11)
+    runs = {}
+    for reuse in (0, 1, 60):
+        with _sim(pos, ori, lu_reuse_iters=reuse, **kw) as sim:
+            for _ in range(6):
+                sim.step()
+                st = sim.stats()
+                assert st.last_backward_error <= 1e-11 and st.last_precond == rf.PRECOND_LU
+            runs[reuse] = (sim.state(), st.lu_factorizations, st.lu_reused_solves, st.total_gmres_iters)
+        print("lu_reuse_iters", reuse, "factorisations", runs[reuse][1], "reused", runs[reuse][2],
+              "GMRES iterations", runs[reuse][3])
+    (p0, t0), nlu0, reused0, _ = runs[0]
+    assert nlu0 == 6 and reused0 == 0
+    _, nlu1, reused1, _ = runs[1]
+    assert nlu1 == 6 and reused1 == 0
+    _, nlu60, reused60, _ = runs[60]
+    assert reused60 >= 3 and nlu60 + reused60 == 6
+    for reuse in (1, 60):
+        (p, t), _, _, _ = runs[reuse]
+        assert np.linalg.norm(p - p0) <= 1e-7 * np.linalg.norm(p0)
+        assert np.linalg.norm(t - t0) <= 1e-7 * np.linalg.norm(t0)
+
+
 @pytest.mark.parametrize("nb,fp16", [(0, 0), (100, 0), (24, 0), (0, 1), (100, 1), (13, 1)])
 def test_fp32_factors_of_fp64_matrix(nb, fp16):
     """lu_factor_fp32: FP32 LU of the FP64 matrix (panel width nb, 0 = 128)

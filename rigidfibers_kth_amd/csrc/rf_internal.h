@@ -78,13 +78,14 @@ hipError_t launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_
 hipError_t launch_reduce_splits(const double* parts, int splits, int64_t stride, int n, double* out,
                                 hipStream_t s);
 // out[j] = sum_i V[i + j*ldv] * w[i] for j < k, out[k] = sum_i w[i]^2.
-// parts must hold blocks*(k+1) doubles; counter is a device int (zero on entry, reset on exit).
+// parts must hold kMaxDotBlocks*(k+1) doubles (multidot_blocks(n) <= kMaxDotBlocks); counter is a device int (zero on entry, reset on exit).
 hipError_t launch_multidot(const double* V, int64_t ldv, int k, const double* w, int n, double* parts,
                            unsigned int* counter, double* out, hipStream_t s);
-// w -= V[:, :k] h ; then optionally out = V^T w (k) and |w|^2 as multidot.
-hipError_t launch_orth_update(const double* V, int64_t ldv, int k, const double* h, double* w, int n,
-                              int fused_dots, double* parts, unsigned int* counter, double* out,
-                              hipStream_t s);
+// w -= V[:, :k] h, then out[0..k) = V^T w (with_dots) and out[nd] = |w|^2
+// (nd = k with dots, else 0), bit-identical to the update followed by
+// launch_multidot; one kernel for k <= 32.
+hipError_t launch_orth_dots(const double* V, int64_t ldv, int k, const double* h, double* w, int n, int with_dots,
+                            double* parts, unsigned int* counter, double* out, hipStream_t s);
 hipError_t launch_scale_copy(const double* w, const double* scale_num, int scale_idx, int n, double* v,
                              hipStream_t s);
 // xc = x + V[:, :k] y, y in host memory (passed as kernel arguments)
@@ -93,6 +94,9 @@ hipError_t launch_vy_plus_x(const double* V, int64_t ldv, int k, const double* y
 // dst[0:n] = src[0:n] on the device (a kernel: no runtime copy packet)
 hipError_t launch_copy_vec(const double* src, double* dst, int n, hipStream_t s);
 int multidot_blocks(int n);
+// workgroups per dot kernel at most: partial buffers hold kMaxDotBlocks *
+// (slots) doubles
+constexpr int kMaxDotBlocks = 64;
 
 // Velocity epilogue, advance (rf_step.hip).
 hipError_t launch_build_fext(int N, int M, const double* force, const double* x, double* fext, hipStream_t s);

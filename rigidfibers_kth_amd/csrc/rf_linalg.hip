@@ -196,6 +196,76 @@ __global__ void orth_update_kernel(const double* __restrict__ V, int64_t ldv, in
     w[i] = s;
 }
 
+// Fused w -= V[:, :k] h and the dots that follow it: out[j] = V[:, j]^T w for
+// j < nd (nd = k or 0) and out[nd] = w^T w, with w updated in the same pass.
+// Rows, partial sums and the last workgroup's fixed-order reduction follow
+// orth_update_kernel then multidot_kernel exactly (same grid-stride rows per
+// thread, same wave and block sums, same partial layout), so the results are
+// bit-identical to the two-kernel sequence with one launch less.  k <= MAXK.
+template <int MAXK>
+__global__ void __launch_bounds__(kDotThreads) orth_dots_kernel(const double* __restrict__ V, int64_t ldv, int k,
+                                                                const double* __restrict__ h,
+                                                                double* __restrict__ w, int n, int nd,
+                                                                double* __restrict__ parts,
+                                                                unsigned int* counter, double* __restrict__ out)
+{
+    __shared__ double hs[MAXK];
+    __shared__ double sh[MAXK + 1][kDotThreads / 64];
+    __shared__ bool last;
+    if (threadIdx.x < MAXK) hs[threadIdx.x] = (threadIdx.x < k) ? h[threadIdx.x] : 0.0;
+    __syncthreads();
+    double acc[MAXK + 1];
+#pragma unroll
+    for (int j = 0; j <= MAXK; ++j) acc[j] = 0.0;
+    const int nbx = gridDim.x;
+    for (int i = blockIdx.x * kDotThreads + threadIdx.x; i < n; i += nbx * kDotThreads) {
+        double v[MAXK];
+#pragma unroll
+        for (int j = 0; j < MAXK; ++j) v[j] = (j < k) ? V[(int64_t)j * ldv + i] : 0.0;
+        double x = w[i];
+#pragma unroll
+        for (int j = 0; j < MAXK; ++j)
+            if (j < k) x -= v[j] * hs[j];
+        w[i] = x;
+#pragma unroll
+        for (int j = 0; j < MAXK; ++j)
+            if (j < nd) acc[j] += v[j] * x;
+        acc[MAXK] += x * x;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j <= MAXK; ++j) {
+        if (j < nd || j == MAXK) {
+            const double t = wave_sum(acc[j]);
+            if (lane == 0) sh[j][wv] = t;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x <= nd) {  // slot nd carries the norm
+        const int src = (threadIdx.x == nd) ? MAXK : threadIdx.x;
+        double t = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sh[src][q];
+        parts[(int64_t)threadIdx.x * nbx + blockIdx.x] = t;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int prev = atomicAdd(counter, 1u);
+        last = (prev == (unsigned int)nbx - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        for (int jj = threadIdx.x; jj <= nd; jj += kDotThreads) {
+            double t = 0.0;
+            for (int b = 0; b < nbx; ++b)
+                t += __hip_atomic_load(&parts[(int64_t)jj * nbx + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            out[jj] = t;
+        }
+        if (threadIdx.x == 0) atomicExch(counter, 0u);
+    }
+}
+
 // v = w / sqrt(nrm2[idx])
 __global__ void scale_copy_kernel(const double* __restrict__ w, const double* __restrict__ nrm2, int idx, int n,
                                   double* __restrict__ v)
@@ -336,7 +406,7 @@ hipError_t launch_reduce_splits(const double* parts, int splits, int64_t stride,
 
 int multidot_blocks(int n)
 {
-    return std::max(1, std::min(64, (n + 4 * kDotThreads - 1) / (4 * kDotThreads)));
+    return std::max(1, std::min(kMaxDotBlocks, (n + 4 * kDotThreads - 1) / (4 * kDotThreads)));
 }
 
 hipError_t launch_multidot(const double* V, int64_t ldv, int k, const double* w, int n, double* parts,
@@ -347,13 +417,23 @@ hipError_t launch_multidot(const double* V, int64_t ldv, int k, const double* w,
     return hipGetLastError();
 }
 
-hipError_t launch_orth_update(const double* V, int64_t ldv, int k, const double* h, double* w, int n,
-                              int fused_dots, double* parts, unsigned int* counter, double* out, hipStream_t s)
+hipError_t launch_orth_dots(const double* V, int64_t ldv, int k, const double* h, double* w, int n, int with_dots,
+                            double* parts, unsigned int* counter, double* out, hipStream_t s)
 {
+    const int nd = with_dots ? k : 0;
+    const dim3 grid(multidot_blocks(n)), block(kDotThreads);
+    if (k <= 16) {
+        hipLaunchKernelGGL(orth_dots_kernel<16>, grid, block, 0, s, V, ldv, k, h, w, n, nd, parts, counter, out);
+        return hipGetLastError();
+    }
+    if (k <= 32) {
+        hipLaunchKernelGGL(orth_dots_kernel<32>, grid, block, 0, s, V, ldv, k, h, w, n, nd, parts, counter, out);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(orth_update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, V, ldv, k, h, w, n);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !fused_dots) return e;
-    return launch_multidot(V, ldv, k, w, n, parts, counter, out, s);
+    if (e != hipSuccess) return e;
+    return launch_multidot(with_dots ? V : nullptr, ldv, nd, w, n, parts, counter, out, s);
 }
 
 hipError_t launch_scale_copy(const double* w, const double* nrm2, int idx, int n, double* v, hipStream_t s)

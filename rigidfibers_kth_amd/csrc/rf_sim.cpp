@@ -171,7 +171,7 @@ int rf_sim::alloc_all()
     ALLOC(d_r, (size_t)lda);
     ALLOC(d_xc, (size_t)lda);
     ALLOC(d_gparts, std::max(gemv_parts_elems(planA), gemv_parts_elems(planW)));
-    ALLOC(d_dparts, (size_t)multidot_blocks(n) * (m + 2) + 64);
+    ALLOC(d_dparts, (size_t)kMaxDotBlocks * (m + 2) + 64);
     ALLOC(d_dots, (size_t)(3 * (m + 2) + 8));
     ALLOC(d_counter, 1);
 #undef ALLOC
@@ -396,9 +396,8 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
             if (!w_ready && (rc = matvec(jH))) return rc;
             w_ready = false;
             HIP_TRY(launch_multidot(d_V, ldv, k, d_w, n, d_dparts, d_counter, D1, stream));
-            HIP_TRY(launch_orth_update(d_V, ldv, k, D1, d_w, n, 1, d_dparts, d_counter, D2, stream));
-            HIP_TRY(launch_orth_update(d_V, ldv, k, D2, d_w, n, 0, nullptr, nullptr, nullptr, stream));
-            HIP_TRY(launch_multidot(nullptr, 0, 0, d_w, n, d_dparts, d_counter, D3, stream));
+            HIP_TRY(launch_orth_dots(d_V, ldv, k, D1, d_w, n, 1, d_dparts, d_counter, D2, stream));
+            HIP_TRY(launch_orth_dots(d_V, ldv, k, D2, d_w, n, 0, d_dparts, d_counter, D3, stream));
             if (k < m) HIP_TRY(launch_scale_copy(d_w, D3, 0, n, d_V + (size_t)k * ldv, stream));
             // The Hessenberg column goes to the host; the next basis vector is
             // already on the device, so its product is queued before the host

@@ -193,6 +193,24 @@ def test_auto_preconditioner_starts_on_the_lu_after_a_switch():
     assert nbr[0] == 5 and nbr[1:8] == [0] * 7 and nbr[8] == 5 and nbr[9] == 0, nbr
 
 
+def test_gmres_long_restart_matches_direct():
+    """GMRES(40): the fused update-and-dots kernel of the Gram-Schmidt steps
+    serves k <= 32 basis vectors and the two-kernel path the rest; both reach
+    the direct solver's solution."""
+    pos, ori = _hard_state()
+    with _sim(pos, ori, use_direct_solver=1) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        x64 = sim.coefficients()
+    with _sim(pos, ori, gmres_restart=40, gmres_tol=1e-11, warm_start=rf.WARM_ZERO) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        xg, st = sim.coefficients(), sim.stats()
+    print("GMRES(40) iterations", st.last_gmres_iters)
+    assert st.last_backward_error <= 1e-11 and st.last_gmres_iters > 40
+    assert np.linalg.norm(xg - x64) <= 1e-8 * np.linalg.norm(x64)
+
+
 @pytest.mark.parametrize("precond", [rf.PRECOND_LU, rf.PRECOND_AUTO])
 def test_lu_reuse_across_steps(precond):
     """lu_reuse_iters: the factors of an earlier step's matrix precondition

@@ -42,7 +42,8 @@ extern "C" {
 
 /* precision modes */
 #define RF_PREC_FP64 0  /* FP64 assembly, FP64 matrix, FP64 solve (reference) */
-#define RF_PREC_MIXED 1 /* FP32 matrix (FP64 near field), FP64 residual/Arnoldi */
+#define RF_PREC_MIXED 1 /* FP32 matrix (FP64 near field) and FP32 LU; solves refined against the FP64
+                           residual of the matrix-free operator; FP64 Arnoldi and velocities */
 
 /* initial guess for GMRES */
 #define RF_WARM_PREV 0    /* previous step's coefficients (fortran/advect_fibers.f90:464) */
@@ -89,7 +90,8 @@ typedef struct rf_params {
     int32_t warm_start; /* RF_WARM_* */
     int32_t profile;    /* 1: time every GEMV / assembly launch with HIP events */
     int32_t lu_block;   /* LU panel width (0 = default) */
-    double mixed_inner_tol; /* mixed mode: tolerance of the FP32-operator solve (0 = default 1e-10) */
+    double mixed_inner_tol; /* mixed mode: relative tolerance of each inner solve on the FP32 matrix between
+                               two FP64 residuals of the matrix-free operator (0 = default 1e-6) */
     int32_t lu_valu_update; /* 1: LU trailing update on the vector ALUs instead of the matrix cores */
     int32_t precond;        /* RF_PRECOND_*: right preconditioner of GMRES */
     int32_t precond_neighbours; /* K nearest fibers per preconditioner cluster (1..6, 0 = 4) */
@@ -103,11 +105,6 @@ typedef struct rf_params {
     int32_t lu_update_fp16;       /* 1: FP32 factors (mixed mode or lu_factor_fp32) that precondition GMRES
                                      take trailing updates on fp16-rounded operands (FP32 accumulation);
                                      the direct solver's factors keep FP32 updates */
-    int32_t lu_reuse_iters;       /* RF_PRECOND_LU / RF_PRECOND_AUTO: > 0 lets the LU factors of an earlier
-                                     step's matrix precondition a solve for up to this many GMRES
-                                     iterations; a solve still above the tolerance then factorises the
-                                     current matrix and continues from the iterate reached (0: only
-                                     factors of the current matrix) */
     double refine_tol;            /* GMRES tolerance once the LU preconditioner is in use (0 = gmres_tol) */
 } rf_params;
 
@@ -138,7 +135,10 @@ typedef struct rf_stats {
     int32_t last_precond;           /* RF_PRECOND_* in use at the end of the last GMRES solve */
     int32_t last_neighbour_iters;   /* iterations of the last GMRES solve under the neighbour preconditioner */
     int64_t lu_panel_fallbacks;     /* LU panels redone with the pivot search over whole columns */
-    int64_t lu_reused_solves;       /* GMRES solves that converged on the factors of an earlier matrix */
+    int64_t operator_launches;      /* matrix-free operator evaluations (profile: timed) */
+    double operator_ms_total;
+    int32_t last_refinements;       /* mixed mode: FP64 residuals of the last solve's refinement */
+    int32_t reserved_i;
     double reserved[2];
 } rf_stats;
 
@@ -152,8 +152,7 @@ void rf_params_default(rf_params *p);
  * Lines after the last mandatory one may hold "key value" extensions:
  * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1,
  * lu_block NB, precond none|neighbour|lu|auto, precond_neighbours K, precond_switch_iters I,
- * lu_factor_fp32 0|1, lu_column_panel 0|1, lu_update_fp16 0|1, lu_reuse_iters I, refine_tol T,
- * mixed_inner_tol T. */
+ * lu_factor_fp32 0|1, lu_column_panel 0|1, lu_update_fp16 0|1, refine_tol T, mixed_inner_tol T. */
 int rf_parse_parameters(const char *path, rf_params *out);
 
 /* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per
@@ -240,6 +239,52 @@ int rf_synchronize(rf_sim *sim);
 
 /* y = A x with the assembled device matrix (FP64 arithmetic), n doubles. */
 int rf_matvec(rf_sim *sim, const double *x, double *y);
+
+/* The matrix-free FP64 operator at the current configuration, without the
+ * stored matrix: mat_prod (fortran/matrix_assembly.f90:159-265, numeric inner
+ * integrals), here for both the numeric and the analytic inner integrals of
+ * rf_params.use_analytical_integration, plus the force term of the
+ * right-hand side (fortran/matrix_assembly.f90:632-905) and of the
+ * velocities (fortran/coeffs_and_vels.f90:543-746).
+ *   RF_OP_AX        y = A x                  (x, y: n doubles)
+ *   RF_OP_RESIDUAL  y = b - A x              (x, y: n doubles)
+ *   RF_OP_RHS       y = b                    (x unused, may be NULL; y: n doubles)
+ *   RF_OP_VELOCITY  the velocities for the coefficients x: y[0:3N] = V, y[3N:6N] = Omega
+ *                   (fiber-major xyz, as rf_get_velocities) */
+#define RF_OP_AX 0
+#define RF_OP_RESIDUAL 1
+#define RF_OP_RHS 2
+#define RF_OP_VELOCITY 3
+int rf_operator(rf_sim *sim, int32_t mode, const double *x, double *y);
+
+/* Restart.  Everything the next step reads, as the two-level scheme of
+ * fortran/advect_fibers.f90:498-523 needs it: the current positions and
+ * orientations (X^n, t^n), those of the previous level (X^{n-1}, t^{n-1}),
+ * the velocities of the last step (V^{n-1}, Omega^{n-1}; the multistep
+ * update's old level) and the last solved coefficients (the next initial
+ * guess, fortran/advect_fibers.f90:464).  The reference saves these in
+ * XcTData / VelRotData / Coeffvec (fortran/advect_fibers.f90:395-424,
+ * 546-591) but has no reader.  rf_get_restart_state copies them out (3N or n
+ * doubles each; at steps_taken = 0 the previous level equals the current one
+ * and the velocities are zero); rf_restore_state loads them into a handle of
+ * the same N and parameters, so the next rf_step continues the run: bit for
+ * bit with the previous-step or zero initial guess.  pos_prev, orient_prev,
+ * vel, rot may be NULL when steps_taken = 0; coeffs NULL means zero. */
+int rf_get_restart_state(const rf_sim *sim, int64_t *steps_taken, double *pos, double *orient, double *pos_prev,
+                         double *orient_prev, double *vel, double *rot, double *coeffs);
+int rf_restore_state(rf_sim *sim, int64_t steps_taken, const double *pos, const double *orient,
+                     const double *pos_prev, const double *orient_prev, const double *vel, const double *rot,
+                     const double *coeffs);
+
+/* Reader of the snapshot files (fortran/advect_fibers.f90:395-424,546-591:
+ * '(2F24.16)' one value per line, a blank line after each 3-vector or
+ * coefficient block): record `record` (0-based; -1 = the last) of
+ * values_per_record numbers into out (may be NULL), *num_records = the number
+ * of complete records in the file.  XcTData and VelRotData records hold 6N
+ * values (per fiber x y z then t_x t_y t_z, resp. V then Omega), Coeffvec
+ * records n values. */
+int rf_read_snapshot(const char *path, int64_t values_per_record, int32_t record, double *out,
+                     int32_t *num_records);
 
 /* POS.out ORIENT.out TRANSVel.out ROTVel.out XVec.out BVec.out of the last
  * step into dir, RF_FMT_F16_8 or RF_FMT_F24_16. */

@@ -54,6 +54,8 @@ def lib():
                                              _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_assemble_rhs_rows.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                             _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, _dp]
+        L.orc_apply_rows.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                     ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, _ip, _dp]
         L.orc_compute_velocities_rows.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                   _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
                                                   ctypes.c_int, _ip, _dp, _dp]
@@ -180,6 +182,18 @@ def assemble_rhs_rows(analytic, eps, pos, orient, force, M, noqi, targets):
     lib().orc_assemble_rhs_rows(int(analytic), eps, N, M, _p(pos), _p(orient), _p(force), pv.size, _p(pv),
                                 _p(wv), _p(lv), t.size, t.ctypes.data_as(_ip), _p(b))
     return b
+
+
+def apply_rows(analytic, eps, pos, orient, M, noqi, x, targets):
+    """Rows (A x)_i of the target fibers (3M values each), without forming A."""
+    pos, orient, x = _f64(pos).ravel(), _f64(orient).ravel(), _f64(x).ravel()
+    N = pos.size // 3
+    pv, wv, lv = _setup(M, noqi)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    y = np.empty(3 * M * t.size)
+    lib().orc_apply_rows(int(analytic), eps, N, M, _p(pos), _p(orient), pv.size, _p(pv), _p(wv), _p(lv), _p(x),
+                         t.size, t.ctypes.data_as(_ip), _p(y))
+    return y
 
 
 def compute_velocities_rows(analytic, eps, pos, orient, force, M, noqi, coeff, targets):

@@ -412,6 +412,66 @@ void orc_assemble_matrix(int analytic, double eps, int N, int M,
     }
 }
 
+/* Rows of A x for the target fibers: the identity plus, against every
+ * source fiber, the blocks of fortran/matrix_assembly.f90:345-395 (the same
+ * G tensors and write_block as orc_assemble_matrix), without forming A.  A
+ * checker of backward errors on systems too large for the dense matrix. */
+void orc_apply_rows(int analytic, double eps, int N, int M,
+                    const double *pos, const double *orient,
+                    int LQ, const double *pv, const double *wv, const double *lvec,
+                    const double *x, int ntargets, const int *targets, double *y)
+{
+    consts_t K;
+    make_consts(eps, M, &K);
+    const int m = 3 * M;
+#pragma omp parallel
+    {
+        double *Gall = (double *)malloc(sizeof(double) * LQ * 6 * (M + 1));
+        double *Gmat = (double *)malloc(sizeof(double) * LQ * 6);
+        double *blk = (double *)malloc(sizeof(double) * m * m);
+#pragma omp for schedule(dynamic, 1)
+        for (int ti = 0; ti < ntargets; ++ti) {
+            const int filno = targets[ti];
+            const double *xc = pos + 3 * filno, *ta = orient + 3 * filno;
+            double *yr = y + (size_t)m * ti;
+            for (int r = 0; r < m; ++r) yr[r] = x[(size_t)filno * m + r];
+            for (int fno = 0; fno < N; ++fno) {
+                if (fno == filno) continue;
+                const double *xcb = pos + 3 * fno, *tb = orient + 3 * fno;
+                if (analytic) {
+                    for (int p = 0; p < LQ; ++p) {
+                        double xbar[3] = {xc[0] + pv[p] * ta[0], xc[1] + pv[p] * ta[1],
+                                          xc[2] + pv[p] * ta[2]};
+                        g_analytic_all(xcb, tb, xbar, eps, M, Gall + (size_t)p * 6 * (M + 1));
+                    }
+                }
+                for (int l = 1; l <= M; ++l) {
+                    for (int p = 0; p < LQ; ++p) {
+                        if (analytic) {
+                            memcpy(Gmat + 6 * p, Gall + (size_t)p * 6 * (M + 1) + 6 * l, 6 * sizeof(double));
+                        } else {
+                            double xbar[3] = {xc[0] + pv[p] * ta[0], xc[1] + pv[p] * ta[1],
+                                              xc[2] + pv[p] * ta[2]};
+                            orc_G_numeric(xcb, tb, xbar, eps, LQ, pv, wv, lvec + (l - 1) * LQ, Gmat + 6 * p);
+                        }
+                    }
+                    /* the (filno, fno) block as a local m x m matrix */
+                    write_block(blk, m, M, LQ, wv, lvec, &K, 0, 0, l, Gmat, ta);
+                }
+                const double *xs = x + (size_t)fno * m;
+                for (int r = 0; r < m; ++r) {
+                    double acc = 0.0;
+                    for (int c = 0; c < m; ++c) acc += blk[r + (size_t)c * m] * xs[c];
+                    yr[r] += acc;
+                }
+            }
+        }
+        free(Gall);
+        free(Gmat);
+        free(blk);
+    }
+}
+
 void orc_assemble_rhs(int analytic, double eps, int N, int M,
                       const double *pos, const double *orient, const double *force,
                       int LQ, const double *pv, const double *wv,

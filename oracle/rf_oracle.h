@@ -58,6 +58,12 @@ void orc_assemble_rhs_rows(int analytic, double eps, int N, int M,
                            const double *pos, const double *orient, const double *force,
                            int LQ, const double *pv, const double *wv,
                            const double *lvec, int ntargets, const int *targets, double *b);
+/* Rows of y = A x for the fibers targets[0..ntargets) (y: 3M*ntargets),
+ * block by block without forming A (analytic or numeric inner integrals). */
+void orc_apply_rows(int analytic, double eps, int N, int M,
+                    const double *pos, const double *orient,
+                    int LQ, const double *pv, const double *wv, const double *lvec,
+                    const double *x, int ntargets, const int *targets, double *y);
 /* Matrix-free y = A x (numeric kernel): fortran/matrix_assembly.f90:159-265. */
 void orc_mat_prod(double eps, int N, int M, const double *pos, const double *orient,
                   int LQ, const double *pv, const double *wv, const double *lvec,

@@ -27,6 +27,7 @@ WARM_PREV, WARM_ZERO, WARM_EXTRAP, WARM_LEARNED = 0, 1, 2, 3
 FMT_F16_8, FMT_F24_16 = 0, 1
 SOLVER_GMRES, SOLVER_LU, SOLVER_LU_FALLBACK = 0, 1, 2
 PRECOND_NONE, PRECOND_NEIGHBOUR, PRECOND_LU, PRECOND_AUTO = 0, 1, 2, 3
+OP_AX, OP_RESIDUAL, OP_RHS, OP_VELOCITY = 0, 1, 2, 3
 
 
 class rf_params(ctypes.Structure):
@@ -59,7 +60,6 @@ class rf_params(ctypes.Structure):
         ("lu_factor_fp32", ctypes.c_int32),
         ("lu_column_panel", ctypes.c_int32),
         ("lu_update_fp16", ctypes.c_int32),
-        ("lu_reuse_iters", ctypes.c_int32),
         ("refine_tol", ctypes.c_double),
     ]
 
@@ -96,12 +96,15 @@ class rf_stats(ctypes.Structure):
         ("last_precond", ctypes.c_int32),
         ("last_neighbour_iters", ctypes.c_int32),
         ("lu_panel_fallbacks", ctypes.c_int64),
-        ("lu_reused_solves", ctypes.c_int64),
+        ("operator_launches", ctypes.c_int64),
+        ("operator_ms_total", ctypes.c_double),
+        ("last_refinements", ctypes.c_int32),
+        ("reserved_i", ctypes.c_int32),
         ("reserved", ctypes.c_double * 2),
     ]
 
     def as_dict(self):
-        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "reserved"}
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if not f[0].startswith("reserved")}
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -144,6 +147,7 @@ _SIGNATURES = {
                                            ctypes.c_int64, _dp, ctypes.c_int64]),
     "rf_set_coefficients": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "rf_matvec": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "rf_operator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, _dp]),
     "rf_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "rf_get_neighbours": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]),
     "rf_precond_apply": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
@@ -157,6 +161,10 @@ _SIGNATURES = {
     "rf_mlp_train": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                     ctypes.c_float, ctypes.c_uint64, ctypes.c_int32, _fp]),
     "rf_write_step_outputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32]),
+    "rf_get_restart_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)] + [_dp] * 7),
+    "rf_restore_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64] + [_dp] * 7),
+    "rf_read_snapshot": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int32, _dp,
+                                        ctypes.POINTER(ctypes.c_int32)]),
     "rf_selftest_mfma_layout": (ctypes.c_int, [ctypes.c_int32, _dp, ctypes.POINTER(ctypes.c_float)]),
     "rf_selftest_lu": (ctypes.c_int, [ctypes.c_int32] * 6 + [_dp, ctypes.POINTER(ctypes.c_int32),
                                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
@@ -235,6 +243,19 @@ def selftest_copy(nbytes, reps=10):
     ms = ctypes.c_double(0.0)
     _check(lib().rf_selftest_copy(int(nbytes), int(reps), ctypes.byref(ms)), "rf_selftest_copy")
     return ms.value
+
+
+def read_snapshot(path, values_per_record, record=-1):
+    """(record values, number of complete records) of an XcTData /
+    VelRotData / Coeffvec file (fortran/advect_fibers.f90:395-424,546-591);
+    record -1 is the last."""
+    nrec = ctypes.c_int32(0)
+    _check(lib().rf_read_snapshot(os.fsencode(path), int(values_per_record), -1, None, ctypes.byref(nrec)),
+           "rf_read_snapshot")
+    out = np.empty(int(values_per_record))
+    _check(lib().rf_read_snapshot(os.fsencode(path), int(values_per_record), int(record), _p(out),
+                                  ctypes.byref(nrec)), "rf_read_snapshot")
+    return out, int(nrec.value)
 
 
 def default_params(**kw):
@@ -386,6 +407,27 @@ class Simulation:
         _check(lib().rf_matvec(self._h, _p(x), _p(y)), "rf_matvec")
         return y
 
+    def operator(self, mode, x=None):
+        """The matrix-free FP64 operator (rf_operator): OP_AX -> A x,
+        OP_RESIDUAL -> b - A x, OP_RHS -> b, OP_VELOCITY -> (V, Omega) of the
+        coefficients x."""
+        if x is None:
+            if mode != OP_RHS:
+                raise ValueError("x is needed for this mode")
+            xp = None
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            xp = _p(x)
+        y = np.empty(6 * self.N if mode == OP_VELOCITY else self.n)
+        _check(lib().rf_operator(self._h, mode, xp, _p(y)), "rf_operator")
+        if mode == OP_VELOCITY:
+            return y[:3 * self.N].copy(), y[3 * self.N:].copy()
+        return y
+
+    def mat_prod(self, x):
+        """A x without the stored matrix (mat_prod, fortran/matrix_assembly.f90:159-265)."""
+        return self.operator(OP_AX, x)
+
     def neighbours(self, k):
         nbr = np.empty(self.N * k, dtype=np.int32)
         dist = np.empty(self.N * k)
@@ -445,6 +487,27 @@ class Simulation:
         _check(lib().rf_mlp_train(self._h, _f(X), _f(Y), X.shape[0], epochs, batch, lr, seed, int(init), _f(hist)),
                "rf_mlp_train")
         return hist
+
+    # -- restart (the reference saves these levels but has no reader) ---------
+
+    def restart_state(self):
+        """Everything the next step reads: steps taken, current and previous
+        positions / orientations, last velocities, last coefficients."""
+        st = ctypes.c_int64(0)
+        a = {k: np.empty(3 * self.N) for k in ("pos", "orient", "pos_prev", "orient_prev", "vel", "rot")}
+        a["coeffs"] = np.empty(self.n)
+        _check(lib().rf_get_restart_state(self._h, ctypes.byref(st), _p(a["pos"]), _p(a["orient"]),
+                                          _p(a["pos_prev"]), _p(a["orient_prev"]), _p(a["vel"]), _p(a["rot"]),
+                                          _p(a["coeffs"])), "rf_get_restart_state")
+        a["steps_taken"] = int(st.value)
+        return a
+
+    def restore_state(self, state):
+        """Loads a restart_state() dict (or one read from snapshot files)."""
+        args = [np.ascontiguousarray(state[k], dtype=np.float64) if state.get(k) is not None else None
+                for k in ("pos", "orient", "pos_prev", "orient_prev", "vel", "rot", "coeffs")]
+        ptrs = [_p(a) if a is not None else None for a in args]
+        _check(lib().rf_restore_state(self._h, int(state["steps_taken"]), *ptrs), "rf_restore_state")
 
     def synchronize(self):
         _check(lib().rf_synchronize(self._h), "rf_synchronize")

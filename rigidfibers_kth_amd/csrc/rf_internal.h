@@ -46,12 +46,30 @@ struct AssembleArgs {
     const double* pv = nullptr;   // device copy of Tables::pv
     void* A = nullptr;            // double* (FP64) or float* (mixed)
     double* bpart = nullptr;      // [splits][n]
-    void* W = nullptr;            // [3(M+1)N cols][ldw rows], double or float
+    void* W = nullptr;            // [3(M+1)N cols][ldw rows], double or float; nullptr: not written
     int mixed = 0;                // 1: A and W stored as float
     double near_c = 10.0;         // mixed: (pair, point) with c < near_c computed in FP64
 };
 
 hipError_t launch_assemble(const Tables& t, const AssembleArgs& a, hipStream_t s);
+
+// Matrix-free FP64 operator (rf_operator.hip).  Modes:
+//   kOpAx       y = A x                         (n doubles)
+//   kOpResidual y = b - A x                     (n doubles)
+//   kOpRhs      y = b                           (n doubles; x unused)
+//   kOpVelocity y = the velocity sums (IF_0, IF_1) per fiber, 6N doubles, the
+//               product of the velocity operator W with [F/2, a^1..a^M]
+enum { kOpAx = 0, kOpResidual = 1, kOpRhs = 2, kOpVelocity = 3 };
+struct OperatorPlan {
+    int N = 0, M = 0, LQ = 0, analytic = 1;
+    int T = 1, threads = 64, splits = 1, src_per_split = 1, src_stride = 4;
+};
+OperatorPlan plan_operator(int N, int M, int LQ, int analytic, int num_cus);
+// doubles of device scratch one launch needs (source vectors + split partials)
+size_t operator_scratch_doubles(const OperatorPlan& p);
+hipError_t launch_operator(const OperatorPlan& p, const Tables& t, int mode, const double* pos, const double* orient,
+                           const double* force, const double* x, const double* wp, const double* pv, double* scratch,
+                           double* y, hipStream_t s);
 
 // Column-major GEMV y = A x with FP64 arithmetic; A is double or float.
 // Writes partial sums to parts[splits][rows_padded] then reduces into y.

@@ -1,5 +1,6 @@
 // rf_io.cpp -- the reference's file formats: the line-ordered v1 run
 // parameter file and the XcT_init layout file.
+#include <algorithm>
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -174,8 +175,6 @@ extern "C" int rf_parse_parameters(const char* path, rf_params* out)
             ok = parse_int(val, &p.lu_column_panel);
         } else if (key == "lu_update_fp16") {
             ok = parse_int(val, &p.lu_update_fp16);
-        } else if (key == "lu_reuse_iters") {
-            ok = parse_int(val, &p.lu_reuse_iters) && p.lu_reuse_iters >= 0;
         } else if (key == "refine_tol") {
             ok = parse_double(val, &p.refine_tol);
         } else if (key == "mixed_inner_tol") {
@@ -248,3 +247,48 @@ extern "C" int rf_parse_layout(const char* path, int32_t* n, double** pos, doubl
 }
 
 extern "C" void rf_free(void* p) { std::free(p); }
+
+// Snapshot files of fortran/advect_fibers.f90:395-424,546-591: '(2F24.16)'
+// one value per line, blank lines between 3-vectors and after each
+// coefficient block.  Records are consecutive groups of values_per_record
+// numbers; blank lines carry no record boundary of their own.
+extern "C" int rf_read_snapshot(const char* path, int64_t values_per_record, int32_t record, double* out,
+                                int32_t* num_records)
+{
+    if (!path || !num_records || values_per_record < 1 || record < -1) {
+        rf::set_error("rf_read_snapshot: null argument, values_per_record < 1 or record < -1");
+        return RF_EINVAL;
+    }
+    std::ifstream f(path);
+    if (!f) {
+        rf::set_error(std::string("rf_read_snapshot: cannot open ") + path);
+        return RF_EIO;
+    }
+    std::vector<double> cur;
+    std::vector<double> want;
+    cur.reserve((size_t)values_per_record);
+    int32_t complete = 0;
+    std::string tok;
+    while (f >> tok) {
+        double v = 0.0;
+        if (!parse_double(tok, &v)) {
+            rf::set_error(std::string("rf_read_snapshot: bad number '") + tok + "' in " + path);
+            return RF_EIO;
+        }
+        cur.push_back(v);
+        if ((int64_t)cur.size() == values_per_record) {
+            if (record == -1 || record == complete) want.swap(cur);
+            cur.clear();
+            ++complete;
+        }
+    }
+    *num_records = complete;
+    if (!out) return RF_OK;
+    if (complete == 0 || (record >= 0 && record >= complete)) {
+        rf::set_error(std::string("rf_read_snapshot: ") + path + " holds " + std::to_string(complete) +
+                      " complete records of " + std::to_string(values_per_record) + " values");
+        return RF_EIO;
+    }
+    std::copy(want.begin(), want.end(), out);
+    return RF_OK;
+}

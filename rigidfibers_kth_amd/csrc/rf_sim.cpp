@@ -151,12 +151,15 @@ int rf_sim::alloc_all()
         }
         allocs.push_back(q);
         d_A = q;
-        if (hipMalloc(&q, (size_t)ldw * wcols * es) != hipSuccess) {
-            rf::set_error("hipMalloc of the velocity operator failed");
-            return RF_ENOMEM;
+        // the mixed mode takes its velocities from the matrix-free operator
+        if (!mixed) {
+            if (hipMalloc(&q, (size_t)ldw * wcols * es) != hipSuccess) {
+                rf::set_error("hipMalloc of the velocity operator failed");
+                return RF_ENOMEM;
+            }
+            allocs.push_back(q);
+            d_W = q;
         }
-        allocs.push_back(q);
-        d_W = q;
     }
     ALLOC(d_b, (size_t)lda);
     ALLOC(d_bpart, (size_t)asm_splits * n);
@@ -174,11 +177,14 @@ int rf_sim::alloc_all()
     ALLOC(d_dparts, (size_t)kMaxDotBlocks * (m + 2) + 64);
     ALLOC(d_dots, (size_t)(3 * (m + 2) + 8));
     ALLOC(d_counter, 1);
+    ALLOC(d_op, operator_scratch_doubles(planOp));
+    ALLOC(d_res64, (size_t)lda);
+    ALLOC(d_dx, (size_t)lda);
 #undef ALLOC
     HIP_TRY(hipMemsetAsync(d_counter, 0, sizeof(unsigned int), stream));
     // zero whole buffers once so the padding rows of A and W read as 0
     HIP_TRY(hipMemsetAsync(d_A, 0, (size_t)lda * n * es, stream));
-    HIP_TRY(hipMemsetAsync(d_W, 0, (size_t)ldw * wcols * es, stream));
+    if (d_W) HIP_TRY(hipMemsetAsync(d_W, 0, (size_t)ldw * wcols * es, stream));
     HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * lda, stream));
     HIP_TRY(hipMemsetAsync(d_xprev, 0, sizeof(double) * lda, stream));
     HIP_TRY(hipMemsetAsync(d_b, 0, sizeof(double) * lda, stream));
@@ -242,6 +248,9 @@ void rf_sim::resolve_timed()
             break;
         case kLU:
             stats.lu_ms_total += ms;
+            break;
+        case kOperator:
+            stats.operator_ms_total += ms;
             break;
         }
         ev_free.push_back({t.a, t.b});
@@ -342,8 +351,8 @@ static inline int total_before(int cycle, int m, int k) { return cycle * m + k; 
 int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int maxit)
 {
     const int m = restart;
-    double tol = (mixed && p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : p.gmres_tol;
-    if (prec_ready && prec_kind == RF_PRECOND_LU && p.refine_tol > 0.0) tol = p.refine_tol;
+    double tol = cur_tol;
+    if (!mixed && prec_ready && prec_kind == RF_PRECOND_LU && p.refine_tol > 0.0) tol = p.refine_tol;
     const int ldh = m + 1;
     std::vector<double> H((size_t)(m + 1) * (m + 2), 0.0), g(m + 1, 0.0), cs(m + 1), sn(m + 1), y(m + 1);
     int rc;
@@ -574,7 +583,7 @@ int rf_sim::solve_lu(double* be_out)
         return RF_OK;
     }
     // iterative refinement: x += LU^-1 (b - A x)
-    const double tol = (p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : p.gmres_tol;
+    const double tol = cur_tol;
     HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, stream));
     HIP_TRY(launch_copy_vec(d_b, d_r, n, stream));
     double be = 1.0;
@@ -749,6 +758,11 @@ int rf_sim::setup_precond(int kind)
         d_pinfo = static_cast<int*>(q);
     }
     if ((rc = setup_neighbours())) return rc;
+    if (ras_valid) {
+        prec_kind = RF_PRECOND_NEIGHBOUR;
+        prec_ready = true;
+        return RF_OK;
+    }
     HIP_TRY(hipMemsetAsync(d_pinfo, 0, sizeof(int), stream));
     HIP_TRY(launch_ras_setup(N, ras_k, knn, bs, d_nbr, d_A, mixed ? 4 : 8, lda, d_B, d_pinfo, stream));
     int info = 0;
@@ -762,40 +776,8 @@ int rf_sim::setup_precond(int kind)
     }
     prec_kind = RF_PRECOND_NEIGHBOUR;
     prec_ready = true;
+    ras_valid = true;
     return RF_OK;
-}
-
-// GMRES right-preconditioned by LU factors from the iterate in d_x.  With
-// lu_reuse_iters > 0 the factors of an earlier step's matrix go first (the
-// dynamics change the matrix little per step), for at most that many
-// iterations; a solve still above the tolerance then factorises the current
-// matrix and continues from the iterate reached (d_r already holds its
-// residual).  The stopping test is always on the current matrix.
-int rf_sim::gmres_on_lu(int maxit, int* iters, double* be, int* status)
-{
-    int rc;
-    *iters = 0;
-    *status = 0;
-    if (p.lu_reuse_iters > 0 && lu_have && !lu_factored && d_z) {
-        prec_kind = RF_PRECOND_LU;
-        prec_ready = true;
-        if ((rc = solve_gmres(iters, be, status, std::min(maxit, p.lu_reuse_iters)))) return rc;
-        if (*status == 0) {
-            stats.lu_reused_solves += 1;
-            return RF_OK;
-        }
-        if (*iters >= maxit) return RF_OK;
-        r0_ready = true;
-        const double r_init = stats.last_initial_residual;
-        if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
-        int it2 = 0;
-        if ((rc = solve_gmres(&it2, be, status, maxit - *iters))) return rc;
-        *iters += it2;
-        stats.last_initial_residual = r_init;
-        return RF_OK;
-    }
-    if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
-    return solve_gmres(iters, be, status, maxit);
 }
 
 int rf_sim::precond_apply(const double* r, double* z)
@@ -839,7 +821,7 @@ int rf_sim::assemble()
     a.wp = d_wp;
     a.pv = d_pv;
     a.A = d_A;
-    a.W = d_W;
+    a.W = d_W;  // nullptr in the mixed mode
     a.bpart = d_bpart;
     a.mixed = mixed;
     hipEvent_t ea = nullptr, eb = nullptr;
@@ -857,6 +839,160 @@ int rf_sim::assemble()
     stats.total_ms_assemble += stats.ms_assemble;
     assembled = true;
     lu_factored = false;
+    ras_valid = false;
+    return RF_OK;
+}
+
+// The matrix-free FP64 operator at the current configuration (kOp* modes).
+int rf_sim::apply_operator(int mode, const double* x, double* y)
+{
+    hipEvent_t ea = nullptr, eb = nullptr;
+    int rc;
+    if (p.profile) {
+        if ((rc = timed_pair(&ea, &eb))) return rc;
+        HIP_TRY(hipEventRecord(ea, stream));
+    }
+    HIP_TRY(launch_operator(planOp, tab, mode, d_pos[lvl_three], d_orient[lvl_three], d_force, x, d_wp, d_pv, d_op, y,
+                            stream));
+    if (p.profile) {
+        HIP_TRY(hipEventRecord(eb, stream));
+        timed_push(ea, eb, kOperator);
+    }
+    stats.operator_launches += 1;
+    return RF_OK;
+}
+
+// Solve of the stored system A x = b (d_b, d_x holding the initial guess) to
+// the relative tolerance `tol` with the configured method: the direct
+// solver, or GMRES (optionally preconditioned) with the reference's LU
+// fallback.  *be is the backward error against the stored matrix.
+int rf_sim::solve_system(int maxit, double tol, int* iters_out, double* be_out)
+{
+    int rc;
+    int iters = 0, status = 0;
+    double be = 0.0;
+    cur_tol = tol;
+    if (p.use_direct_solver) {
+        r0_ready = false;
+        if ((rc = solve_lu(&be))) return rc;
+        stats.last_solver = RF_SOLVER_LU;
+        *iters_out = 0;
+        *be_out = be;
+        return RF_OK;
+    }
+    prec_ready = false;
+    prec_kind = RF_PRECOND_NONE;
+    if (p.precond == RF_PRECOND_AUTO) {
+        // Neighbour preconditioner within a budget (two restart cycles by
+        // default), then GMRES on A (LU)^-1 from the iterate reached.  After
+        // the first cycle the observed residual reduction per iteration
+        // predicts the iterations still needed; if they exceed the budget the
+        // switch happens at once.  A solve that ended on the LU makes the next
+        // ones start on it (the dynamics change the matrix little per step),
+        // and every kAutoRetry-th of them tries the neighbour preconditioner
+        // again; factors of this very matrix are always used at once.
+        const int budget = std::min(maxit, p.precond_switch_iters > 0 ? p.precond_switch_iters : 2 * restart);
+        const bool lu_first = lu_factored || (auto_lu_run > 0 && auto_lu_run % kAutoRetry != 0);
+        if (lu_first) {
+            if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
+            if ((rc = solve_gmres(&iters, &be, &status, maxit))) return rc;
+        } else {
+            if ((rc = setup_precond(RF_PRECOND_NEIGHBOUR))) return rc;
+            if ((rc = solve_gmres(&iters, &be, &status, std::min(budget, restart)))) return rc;
+        }
+        const double r0 = stats.last_initial_residual;
+        if (!lu_first && status != 0 && iters < budget && r0 > 0.0 && be > 0.0 && be < r0) {
+            const double rate = std::pow(be / r0, 1.0 / iters);
+            const double needed = std::log(tol / be) / std::log(rate);
+            if (iters + needed <= budget) {
+                int it2 = 0;
+                if ((rc = solve_gmres(&it2, &be, &status, budget - iters))) return rc;
+                iters += it2;
+            }
+        }
+        stats.last_neighbour_iters = lu_first ? 0 : iters;
+        if (!lu_first && status != 0 && iters < maxit) {
+            int it2 = 0;
+            if ((rc = setup_precond(RF_PRECOND_LU))) return rc;
+            if ((rc = solve_gmres(&it2, &be, &status, maxit - iters))) return rc;
+            iters += it2;
+        }
+        stats.last_initial_residual = r0;
+    } else {
+        if (p.precond != RF_PRECOND_NONE && (rc = setup_precond(p.precond))) return rc;
+        if ((rc = solve_gmres(&iters, &be, &status, maxit))) return rc;
+        stats.last_neighbour_iters = (p.precond == RF_PRECOND_NEIGHBOUR) ? iters : 0;
+    }
+    stats.last_precond = prec_kind;
+    stats.last_solver = RF_SOLVER_GMRES;
+    if (status == -4) {  // fortran/coeffs_and_vels.f90:331-335
+        if ((rc = solve_lu(&be))) return rc;
+        stats.last_solver = RF_SOLVER_LU_FALLBACK;
+    }
+    *iters_out = iters;
+    *be_out = be;
+    return RF_OK;
+}
+
+// Mixed precision: iterative refinement of the FP32 matrix's solves against
+// the FP64 residual of the matrix-free operator,
+//     r = b - A x (FP64, no stored matrix);  solve A32 d = r to mixed_inner_tol;  x += d,
+// until ||r|| / ||b|| <= gmres_tol.  Each inner solve uses the configured
+// method on the FP32 matrix (GMRES with FP64 Arnoldi, its preconditioner and
+// FP32 LU factors, or the direct solver).
+int rf_sim::solve_refined(int* iters_out, double* be_out)
+{
+    constexpr int kMaxRefinements = 30;
+    const double tol = p.gmres_tol;
+    const double inner_tol = (p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : 1e-6;
+    const int maxit = p.gmres_max_iters;
+    int rc;
+    double bn2;
+    if ((rc = norm2(d_b, &bn2))) return rc;  // the scale only: FP32-accurate is enough
+    const double bn = std::sqrt(bn2);
+    *iters_out = 0;
+    stats.last_refinements = 0;
+    if (bn == 0.0) {
+        HIP_TRY(launch_fill(d_x, 0.0, (size_t)n, stream));
+        *be_out = 0.0;
+        return RF_OK;
+    }
+    r0_ready = false;  // a residual of the learned guess is on the FP32 matrix
+    int total = 0;
+    double be = 0.0, be_prev = 0.0;
+    int stalls = 0;
+    for (int k = 0;; ++k) {
+        if ((rc = apply_operator(kOpResidual, d_x, d_res64))) return rc;
+        double rr;
+        if ((rc = norm2(d_res64, &rr))) return rc;
+        be = std::sqrt(rr) / bn;
+        stats.last_refinements = k + 1;
+        if (k == 0) stats.last_initial_residual = be;
+        if (be <= tol) break;
+        if (k > 0 && be > 0.5 * be_prev) ++stalls;
+        if (k >= kMaxRefinements || total >= maxit || stalls >= 3) {
+            rf::set_error("mixed precision: refinement against the FP64 operator stalled at backward error " +
+                          std::to_string(be) + " after " + std::to_string(k) + " corrections");
+            *iters_out = total;
+            *be_out = be;
+            return RF_ENOCONV;
+        }
+        be_prev = be;
+        // inner solve A32 d = r from d = 0 on the buffers of the plain solve
+        std::swap(d_b, d_res64);
+        std::swap(d_x, d_dx);
+        int it = 0;
+        double be_in = 0.0;
+        rc = launch_fill(d_x, 0.0, (size_t)n, stream) == hipSuccess ? RF_OK : RF_EHIP;
+        if (rc == RF_OK) rc = solve_system(std::max(1, maxit - total), inner_tol, &it, &be_in);
+        std::swap(d_b, d_res64);
+        std::swap(d_x, d_dx);
+        if (rc) return rc;
+        total += it;
+        HIP_TRY(launch_axpby(n, 1.0, d_x, 1.0, d_dx, d_x, stream));
+    }
+    *iters_out = total;
+    *be_out = be;
     return RF_OK;
 }
 
@@ -870,7 +1006,7 @@ int rf_sim::solve()
     if (rc) return rc;
     // initial guess (fortran/advect_fibers.f90:464: init_guess = coeffvec)
     if (p.warm_start == RF_WARM_ZERO) {
-        HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, stream));
+        HIP_TRY(launch_fill(d_x, 0.0, (size_t)n, stream));
     } else if (p.warm_start == RF_WARM_EXTRAP && solves_done >= 2) {
         HIP_TRY(launch_axpby(n, 2.0, d_x, -1.0, d_xprev, d_guess, stream));
         HIP_TRY(launch_copy_vec(d_x, d_xprev, n, stream));
@@ -880,70 +1016,14 @@ int rf_sim::solve()
         if (p.warm_start == RF_WARM_LEARNED && mlp_weights && !p.use_direct_solver && (rc = learned_guess()))
             return rc;
     }
-    int iters = 0, status = 0;
+    int iters = 0;
     double be = 0.0;
-    if (p.use_direct_solver) {
-        r0_ready = false;
-        rc = solve_lu(&be);
-        if (rc) return rc;
-        stats.last_solver = RF_SOLVER_LU;
-    } else {
-        prec_ready = false;
-        prec_kind = RF_PRECOND_NONE;
-        const int maxit = p.gmres_max_iters;
-        if (p.precond == RF_PRECOND_AUTO) {
-            // Neighbour preconditioner within a budget (two restart cycles by
-            // default), then GMRES on A (LU)^-1 from the iterate reached.  After
-            // the first cycle the observed residual reduction per iteration
-            // predicts the iterations still needed; if they exceed the budget
-            // the switch happens at once.
-            // A solve that ended on the LU makes the next ones start on it
-            // (the dynamics change the matrix little per step); every
-            // kAutoRetry-th of them tries the neighbour preconditioner again.
-            const int budget = std::min(maxit, p.precond_switch_iters > 0 ? p.precond_switch_iters : 2 * restart);
-            const double tol = p.gmres_tol;
-            const bool lu_first = auto_lu_run > 0 && auto_lu_run % kAutoRetry != 0;
-            if (lu_first) {
-                if ((rc = gmres_on_lu(maxit, &iters, &be, &status))) return rc;
-            } else {
-                if ((rc = setup_precond(RF_PRECOND_NEIGHBOUR))) return rc;
-                if ((rc = solve_gmres(&iters, &be, &status, std::min(budget, restart)))) return rc;
-            }
-            const double r0 = stats.last_initial_residual;
-            if (!lu_first && status != 0 && iters < budget && r0 > 0.0 && be > 0.0 && be < r0) {
-                const double rate = std::pow(be / r0, 1.0 / iters);
-                const double needed = std::log(tol / be) / std::log(rate);
-                if (iters + needed <= budget) {
-                    int it2 = 0;
-                    if ((rc = solve_gmres(&it2, &be, &status, budget - iters))) return rc;
-                    iters += it2;
-                }
-            }
-            stats.last_neighbour_iters = lu_first ? 0 : iters;
-            if (!lu_first && status != 0 && iters < maxit) {
-                int it2 = 0;
-                if ((rc = gmres_on_lu(maxit - iters, &it2, &be, &status))) return rc;
-                iters += it2;
-            }
-            stats.last_initial_residual = r0;
-            auto_lu_run = (prec_kind == RF_PRECOND_LU) ? auto_lu_run + 1 : 0;
-        } else {
-            if (p.precond == RF_PRECOND_LU) {
-                if ((rc = gmres_on_lu(maxit, &iters, &be, &status))) return rc;
-            } else {
-                if (p.precond != RF_PRECOND_NONE && (rc = setup_precond(p.precond))) return rc;
-                if ((rc = solve_gmres(&iters, &be, &status, maxit))) return rc;
-            }
-            stats.last_neighbour_iters = (p.precond == RF_PRECOND_NEIGHBOUR) ? iters : 0;
-        }
-        stats.last_precond = prec_kind;
-        stats.last_solver = RF_SOLVER_GMRES;
-        if (status == -4) {  // fortran/coeffs_and_vels.f90:331-335
-            rc = solve_lu(&be);
-            if (rc) return rc;
-            stats.last_solver = RF_SOLVER_LU_FALLBACK;
-        }
-    }
+    if (mixed)
+        rc = solve_refined(&iters, &be);
+    else
+        rc = solve_system(p.gmres_max_iters, p.gmres_tol, &iters, &be);
+    if (rc) return rc;
+    if (!p.use_direct_solver) auto_lu_run = (prec_kind == RF_PRECOND_LU) ? auto_lu_run + 1 : 0;
     stats.last_gmres_iters = iters;
     stats.total_gmres_iters += iters;
     stats.last_backward_error = be;
@@ -966,11 +1046,16 @@ int rf_sim::update_velocities()
     // old=new; new=3-old (fortran/advect_fibers.f90:465-466)
     vel_old = vel_new;
     vel_new = 1 - vel_old;
-    HIP_TRY(launch_build_fext(N, M, d_force, d_x, d_fext, stream));
-    hipEvent_t ea = nullptr, eb = nullptr;
-    if (p.profile && (rc = timed_pair(&ea, &eb))) return rc;
-    HIP_TRY(launch_gemv(planW, d_W, d_fext, d_gparts, d_y6, nullptr, 0, stream, ea, eb));
-    if (p.profile) timed_push(ea, eb, kGemvW);
+    if (mixed) {
+        // FP64 velocity sums from the matrix-free operator (no stored W)
+        if ((rc = apply_operator(kOpVelocity, d_x, d_y6))) return rc;
+    } else {
+        HIP_TRY(launch_build_fext(N, M, d_force, d_x, d_fext, stream));
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (p.profile && (rc = timed_pair(&ea, &eb))) return rc;
+        HIP_TRY(launch_gemv(planW, d_W, d_fext, d_gparts, d_y6, nullptr, 0, stream, ea, eb));
+        if (p.profile) timed_push(ea, eb, kGemvW);
+    }
     HIP_TRY(launch_velocity_epilogue(N, tab.d, tab.e, d_pos[lvl_three], d_orient[lvl_three], d_force, d_y6,
                                      d_vel[vel_new], d_rot[vel_new], stream));
     stats.ms_velocities = phase_end();
@@ -1075,6 +1160,7 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
     const int es = s->mixed ? 4 : 8;
     s->planA = plan_gemv(s->n, s->n, s->lda, es, s->num_cus);
     s->planW = plan_gemv(6 * s->N, s->wcols, s->ldw, es, s->num_cus);
+    s->planOp = plan_operator(s->N, s->M, s->LQ, P.use_analytical_integration ? 1 : 0, s->num_cus);
     // coherent (fine-grained) pinned memory: the GPU's writes into it bypass
     // its caches, so the per-iteration event needs no system-scope fence
     if (hipHostMalloc((void**)&s->h_dots, sizeof(double) * (3 * (s->restart + 2) + 8), hipHostMallocCoherent) !=
@@ -1230,6 +1316,73 @@ extern "C" int rf_set_coefficients(rf_sim* s, const double* x)
     return RF_OK;
 }
 
+// Restart state (the levels the next step reads; see the declaration).
+extern "C" int rf_get_restart_state(const rf_sim* s, int64_t* steps_taken, double* pos, double* orient,
+                                    double* pos_prev, double* orient_prev, double* vel, double* rot, double* coeffs)
+{
+    CHECK_SIM(s);
+    if (!steps_taken || !pos || !orient || !pos_prev || !orient_prev || !vel || !rot || !coeffs) {
+        set_error("rf_get_restart_state: null buffer");
+        return RF_EINVAL;
+    }
+    const size_t N3 = 3 * (size_t)s->N;
+    *steps_taken = s->steps_taken;
+    int rc;
+    if ((rc = copy_out(s, pos, s->d_pos[s->lvl_three], N3))) return rc;
+    if ((rc = copy_out(s, orient, s->d_orient[s->lvl_three], N3))) return rc;
+    if ((rc = copy_out(s, coeffs, s->d_x, (size_t)s->n))) return rc;
+    if (s->steps_taken == 0) {
+        std::memcpy(pos_prev, pos, sizeof(double) * N3);
+        std::memcpy(orient_prev, orient, sizeof(double) * N3);
+        std::memset(vel, 0, sizeof(double) * N3);
+        std::memset(rot, 0, sizeof(double) * N3);
+        return RF_OK;
+    }
+    if ((rc = copy_out(s, pos_prev, s->d_pos[s->lvl_two], N3))) return rc;
+    if ((rc = copy_out(s, orient_prev, s->d_orient[s->lvl_two], N3))) return rc;
+    if ((rc = copy_out(s, vel, s->d_vel[s->vel_new], N3))) return rc;
+    return copy_out(s, rot, s->d_rot[s->vel_new], N3);
+}
+
+extern "C" int rf_restore_state(rf_sim* s, int64_t steps_taken, const double* pos, const double* orient,
+                                const double* pos_prev, const double* orient_prev, const double* vel,
+                                const double* rot, const double* coeffs)
+{
+    CHECK_SIM(s);
+    if (steps_taken < 0 || !pos || !orient ||
+        (steps_taken > 0 && (!pos_prev || !orient_prev || !vel || !rot))) {
+        set_error("rf_restore_state: steps_taken < 0, or a null level the next step needs");
+        return RF_EINVAL;
+    }
+    const size_t b3 = sizeof(double) * 3 * (size_t)s->N;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_pos[s->lvl_three], pos, b3, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->d_orient[s->lvl_three], orient, b3, hipMemcpyHostToDevice, s->stream));
+    if (steps_taken > 0) {
+        HIP_TRY(hipMemcpyAsync(s->d_pos[s->lvl_two], pos_prev, b3, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->d_orient[s->lvl_two], orient_prev, b3, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->d_vel[s->vel_new], vel, b3, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->d_rot[s->vel_new], rot, b3, hipMemcpyHostToDevice, s->stream));
+    }
+    if (coeffs) {
+        HIP_TRY(hipMemcpyAsync(s->d_x, coeffs, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    } else {
+        HIP_TRY(launch_fill(s->d_x, 0.0, (size_t)s->n, s->stream));
+    }
+    HIP_TRY(launch_copy_vec(s->d_x, s->d_xprev, s->n, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->steps_taken = steps_taken;
+    s->stats.steps_taken = steps_taken;
+    s->solves_done = steps_taken;
+    s->assembled = s->solved = s->have_velocities = false;
+    s->prec_ready = false;
+    s->nbr_valid = false;
+    s->ras_valid = false;
+    s->lu_factored = false;
+    s->auto_lu_run = 0;
+    return RF_OK;
+}
+
 extern "C" int rf_get_matrix_block(const rf_sim* s, int64_t row0, int64_t col0, int64_t rows, int64_t cols,
                                    double* out, int64_t ldo)
 {
@@ -1288,6 +1441,27 @@ extern "C" int rf_matvec(rf_sim* s, const double* x, double* y)
     int rc = s->gemv_A(s->d_xc, s->d_r, nullptr, 0);
     if (rc) return rc;
     return copy_out(s, y, s->d_r, (size_t)s->n);
+}
+
+extern "C" int rf_operator(rf_sim* s, int32_t mode, const double* x, double* y)
+{
+    CHECK_SIM(s);
+    if (!y || (mode != RF_OP_RHS && !x) || mode < RF_OP_AX || mode > RF_OP_VELOCITY) {
+        set_error("rf_operator: null buffer or unknown mode");
+        return RF_EINVAL;
+    }
+    int rc;
+    if (mode != RF_OP_RHS) HIP_TRY(hipMemcpyAsync(s->d_xc, x, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    const int kmode = (mode == RF_OP_AX) ? kOpAx : (mode == RF_OP_RESIDUAL) ? kOpResidual : (mode == RF_OP_RHS) ? kOpRhs : kOpVelocity;
+    double* out = (mode == RF_OP_VELOCITY) ? s->d_y6 : s->d_t ? s->d_t : s->d_w;
+    if ((rc = s->apply_operator(kmode, s->d_xc, out))) return rc;
+    if (mode != RF_OP_VELOCITY) return copy_out(s, y, out, (size_t)s->n);
+    // V and Omega of these coefficients (the handle's own velocities stay unchanged)
+    double* v = s->d_r;
+    double* w = s->d_r + 3 * (size_t)s->N;
+    HIP_TRY(launch_velocity_epilogue(s->N, s->tab.d, s->tab.e, s->d_pos[s->lvl_three], s->d_orient[s->lvl_three],
+                                     s->d_force, s->d_y6, v, w, s->stream));
+    return copy_out(s, y, v, 6 * (size_t)s->N);
 }
 
 extern "C" int rf_get_neighbours(rf_sim* s, int32_t k, int32_t* nbr, double* dist)

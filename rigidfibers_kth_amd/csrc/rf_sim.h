@@ -19,6 +19,7 @@ struct rf_sim {
     int asm_splits = 1;
     rf::Tables tab;
     rf::GemvPlan planA, planW;
+    rf::OperatorPlan planOp;
     hipStream_t stream = nullptr;
     hipEvent_t ev_p0 = nullptr, ev_p1 = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
     std::vector<hipEvent_t> events;
@@ -48,6 +49,11 @@ struct rf_sim {
     double* d_guess = nullptr;
     double* d_fext = nullptr;
     double* d_y6 = nullptr;
+    // matrix-free operator: scratch, and the mixed mode's FP64 residual and correction
+    double* d_op = nullptr;
+    double* d_res64 = nullptr;
+    double* d_dx = nullptr;
+    double cur_tol = 1e-10;  // stopping tolerance of the solve in progress
     // GMRES workspace
     double* d_V = nullptr;
     double* d_w = nullptr;
@@ -69,6 +75,7 @@ struct rf_sim {
     double* d_t = nullptr;
     int* d_pinfo = nullptr;
     bool prec_ready = false;
+    bool ras_valid = false;  // d_B holds the neighbour preconditioner of the current matrix
     int prec_kind = 0;       // RF_PRECOND_NEIGHBOUR or RF_PRECOND_LU once prec_ready
     int lu_es = 8;           // element bytes of the LU factors
     bool nbr_valid = false;  // neighbour lists match the current configuration
@@ -97,7 +104,7 @@ struct rf_sim {
 
     // per-kernel timing without synchronisation: events are recorded around
     // launches and resolved at the next point where the stream is idle.
-    enum TimedKind { kGemvA = 0, kAssemble = 1, kGemvW = 2, kLU = 3 };
+    enum TimedKind { kGemvA = 0, kAssemble = 1, kGemvW = 2, kLU = 3, kOperator = 4 };
     struct Timed {
         hipEvent_t a, b;
         int kind;
@@ -119,6 +126,9 @@ struct rf_sim {
     int fetch_dots(int count);
     int norm2(const double* v, double* out);
     int solve_gmres(int* iters, double* be, int* status, int maxit);
+    int solve_system(int maxit, double tol, int* iters, double* be);
+    int solve_refined(int* iters, double* be);
+    int apply_operator(int mode, const double* x, double* y);
     int solve_lu(double* be);
     int factorize(bool* done, bool precond_only);
     int lu_apply(double* x);
@@ -127,7 +137,6 @@ struct rf_sim {
     int mlp_infer(double* x_out);
     int learned_guess();
     int setup_precond(int kind);
-    int gmres_on_lu(int maxit, int* iters, double* be, int* status);
     int precond_apply(const double* r, double* z);
     int update_iterate(int k);
     int phase_begin();

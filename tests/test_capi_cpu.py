@@ -61,13 +61,13 @@ def test_parse_extension_lines(tmp_path):
     assert p.precision == rf.PREC_MIXED and p.warm_start == rf.WARM_LEARNED and p.profile == 1
     f.write_text(src + "precond auto\nprecond_switch_iters 45\nlu_factor_fp32 1\nrefine_tol 1d-13\n"
                        "precond_neighbours 6\nmixed_inner_tol 1e-11\nlu_block 96\nlu_update_fp16 1\n"
-                       "lu_column_panel 1\nlu_reuse_iters 40\n")
+                       "lu_column_panel 1\n")
     p = rf.Parameters.parse_parameter_file(str(f))
     assert (p.precond, p.precond_switch_iters, p.lu_factor_fp32, p.precond_neighbours, p.lu_block) == \
         (rf.PRECOND_AUTO, 45, 1, 6, 96)
-    assert (p.lu_update_fp16, p.lu_column_panel, p.lu_reuse_iters) == (1, 1, 40)
+    assert (p.lu_update_fp16, p.lu_column_panel) == (1, 1)
     assert p.refine_tol == 1e-13 and p.mixed_inner_tol == 1e-11
-    for bad in ("precond sometimes\n", "warm_start later\n", "precision fp16\n", "lu_reuse_iters -1\n"):
+    for bad in ("precond sometimes\n", "warm_start later\n", "precision fp16\n", "lu_reuse_iters 4\n"):
         f.write_text(src + bad)
         with pytest.raises(rf.RigidFibersError):
             rf.Parameters.parse_parameter_file(str(f))
@@ -143,3 +143,28 @@ def test_cli_help_and_version():
     assert "gfx950" in out
     r = subprocess.run([exe, "only-one-arg"], capture_output=True, text=True)
     assert r.returncode == 1
+
+
+def test_read_snapshot_records(tmp_path):
+    """rf_read_snapshot: records of '(2F24.16)' lines with blank separators
+    (fortran/advect_fibers.f90:395-424,546-591), the last by default, an
+    incomplete trailing record ignored, a record beyond the end an error."""
+    N = 2
+    f = tmp_path / "XcTData1_p1"
+    vals = np.arange(3 * 6 * N, dtype=float) * 0.125 - 1.0
+    with open(f, "w") as fh:
+        for r in range(3):
+            for i in range(N):
+                for half in range(2):
+                    for c in range(3):
+                        fh.write("%24.16f\n" % vals[r * 6 * N + 6 * i + 3 * half + c])
+                    fh.write(" \n")
+        fh.write("%24.16f\n" % 7.0)  # incomplete record
+    last, n = rf.read_snapshot(str(f), 6 * N)
+    assert n == 3 and np.array_equal(last, vals[12 * 2:12 * 3])
+    first, _ = rf.read_snapshot(str(f), 6 * N, 0)
+    assert np.array_equal(first, vals[:12])
+    with pytest.raises(rf.RigidFibersError):
+        rf.read_snapshot(str(f), 6 * N, 3)
+    with pytest.raises(rf.RigidFibersError):
+        rf.read_snapshot(str(tmp_path / "missing"), 6 * N)

@@ -1,0 +1,142 @@
+"""Matrix-free FP64 operator (rf_operator; mat_prod of
+fortran/matrix_assembly.f90:159-265) against the oracle and the stored matrix.
+
+The operator never forms A: for every target point it sums the inner
+integrals of all source fibers against their force densities, then
+contracts over the points once.  Checks, all through librigidfibers.so:
+
+  * numeric inner integrals vs the oracle's mat_prod (the reference's own
+    matrix-free routine), and vs A x with the oracle's assembled matrix:
+    <= 1e-12 of max |y|;
+  * both inner integrals vs the GPU's stored FP64 matrix (rf_get_matrix,
+    the same formulas through a different kernel and summation order):
+    <= 1e-12 of max |y|;
+  * analytic inner integrals vs the oracle's assembled matrix: <= 2e-10 of
+    max |y|, the bound of the analytic matrix parity test (the recurrences of
+    fortran/Int_analytical.f90:123-206 amplify last-bit differences);
+  * b, b - A x and the velocities from the operator vs the assembled rhs, the
+    stored matrix and the velocity-operator path: <= 1e-12;
+  * mixed precision: the solve refined against this operator reaches the
+    FP64 tolerance against the true FP64 system, checked row by row against
+    the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import rigidfibers_kth_amd as rf
+from rigidfibers_kth_amd.layouts import layout
+
+pytestmark = pytest.mark.gpu
+EPS = 0.01
+VARIANTS = [(5, 8), (3, 4), (8, 6), (1, 2)]
+
+
+def _params(analytic, **kw):
+    base = dict(use_analytical_integration=int(analytic), use_direct_solver=0, gmres_restart=10,
+                gmres_max_iters=300, gmres_tol=1e-12)
+    base.update(kw)
+    return rf.default_params(**base)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("M,noqi", VARIANTS)
+def test_numeric_operator_vs_oracle_mat_prod(M, noqi):
+    N = 21
+    pos, ori = layout(N, 2e-2, 300 + M)
+    x = np.random.default_rng(M).normal(size=3 * M * N)
+    with rf.Simulation(rf.Configuration(_params(0, num_terms=M, num_quad_intervals=noqi), pos, ori)) as sim:
+        y = sim.mat_prod(x)
+    yo = oracle.mat_prod(EPS, pos, ori, M, noqi, x)
+    Ao = oracle.assemble_matrix(0, EPS, pos, ori, M, noqi)
+    print("M=%d NoQI=%d numeric: vs mat_prod %.2e, vs assembled %.2e" % (M, noqi, _rel(y, yo), _rel(y, Ao @ x)))
+    assert _rel(y, yo) <= 1e-12
+    assert _rel(y, Ao @ x) <= 1e-12
+
+
+@pytest.mark.parametrize("analytic", [1, 0])
+@pytest.mark.parametrize("M,noqi", VARIANTS)
+def test_operator_vs_stored_matrix(analytic, M, noqi):
+    N = 21
+    pos, ori = layout(N, 2e-2, 400 + M)
+    rng = np.random.default_rng(10 * M + analytic)
+    x = rng.normal(size=3 * M * N)
+    with rf.Simulation(rf.Configuration(_params(analytic, num_terms=M, num_quad_intervals=noqi), pos, ori)) as sim:
+        sim.assemble_system()
+        A = sim.matrix()
+        b = sim.rhs()
+        y = sim.operator(rf.OP_AX, x)
+        r = sim.operator(rf.OP_RESIDUAL, x)
+        bo = sim.operator(rf.OP_RHS)
+        yd = sim.matvec(x)
+    Ao = oracle.assemble_matrix(analytic, EPS, pos, ori, M, noqi)
+    e_gpu = _rel(y, A @ x)
+    e_orc = _rel(y, Ao @ x)
+    print("M=%d NoQI=%d analytic=%d: vs stored %.2e, vs GEMV %.2e, vs oracle %.2e; rhs %.2e; residual %.2e" % (
+        M, noqi, analytic, e_gpu, _rel(y, yd), e_orc, _rel(bo, b), _rel(r, b - A @ x)))
+    assert e_gpu <= 1e-12
+    assert _rel(y, yd) <= 1e-12
+    assert e_orc <= (2e-10 if analytic else 1e-12)
+    assert _rel(bo, b) <= 1e-12
+    assert _rel(r, b - A @ x) <= 1e-12
+
+
+@pytest.mark.parametrize("analytic", [1, 0])
+def test_operator_velocities(analytic):
+    """OP_VELOCITY for the solved coefficients equals the stored velocity
+    operator's result and the oracle's compute_velocities."""
+    N, M, noqi = 30, 5, 8
+    pos, ori = layout(N, 1e-2, 7)
+    F = (0.2, -0.1, -1.0)
+    p = _params(analytic, gmres_tol=1e-13, external_force=F)
+    with rf.Simulation(rf.Configuration(p, pos, ori)) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        x = sim.coefficients()
+        sim.update_velocities()
+        v, w = sim.velocities()
+        vo, wo = sim.operator(rf.OP_VELOCITY, x)
+        v2, w2 = sim.velocities()  # the handle's velocities are unchanged
+    vr, wr = oracle.compute_velocities(analytic, EPS, pos, ori, np.tile(F, N), M, noqi, x)
+    print("analytic=%d: V %.2e / %.2e, Omega %.2e / %.2e" % (analytic, _rel(vo, v), _rel(vo, vr), _rel(wo, w),
+                                                                 _rel(wo, wr)))
+    assert np.array_equal(v, v2) and np.array_equal(w, w2)
+    assert _rel(vo, v) <= 1e-12 and _rel(wo, w) <= 1e-12
+    tol = 2e-10 if analytic else 1e-12
+    assert _rel(vo, vr) <= tol and _rel(wo, wr) <= tol
+
+
+def test_mixed_solve_against_fp64_operator():
+    """RF_PREC_MIXED: FP32 matrix and FP32 LU, refined against the FP64
+    residual of the operator.  The solution's backward error against the FP64
+    system (operator residual) meets the tolerance, the velocities come from
+    the FP64 operator, and the result agrees with the FP64 solve."""
+    N = 200
+    pos, ori = layout(N, 2e-2, 11)
+    res = {}
+    for prec in (rf.PREC_FP64, rf.PREC_MIXED):
+        p = _params(1, gmres_restart=30, gmres_tol=1e-10, precision=prec, precond=rf.PRECOND_AUTO,
+                    lu_update_fp16=1)
+        with rf.Simulation(rf.Configuration(p, pos, ori)) as sim:
+            sim.assemble_system()
+            sim.solve_system()
+            x = sim.coefficients()
+            st = sim.stats()
+            r = sim.operator(rf.OP_RESIDUAL, x)
+            b = sim.operator(rf.OP_RHS)
+            sim.update_velocities()
+            v, w = sim.velocities()
+        res[prec] = (x, np.linalg.norm(r) / np.linalg.norm(b), st, v, w)
+        print("precision %d: backward error %.2e (reported %.2e), %d iterations, %d refinements" % (
+            prec, res[prec][1], st.last_backward_error, st.last_gmres_iters, st.last_refinements))
+    x64, be64, _, v64, w64 = res[rf.PREC_FP64]
+    xm, bem, stm, vm, wm = res[rf.PREC_MIXED]
+    assert be64 <= 1e-10 and bem <= 1e-10
+    assert stm.last_backward_error <= 1e-10 and stm.last_refinements >= 2
+    assert abs(stm.last_backward_error - bem) <= 1e-2 * bem + 1e-15
+    # both solve the same FP64 system to 1e-10
+    assert np.linalg.norm(xm - x64) <= 1e-8 * np.linalg.norm(x64)
+    assert _rel(vm, v64) <= 1e-8 and _rel(wm, w64) <= 1e-8

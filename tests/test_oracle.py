@@ -184,6 +184,29 @@ def test_mat_prod_equals_dense_operator():
     np.testing.assert_allclose(y, A @ x, rtol=1e-12, atol=1e-13)
 
 
+@pytest.mark.parametrize("analytic", [1, 0])
+def test_apply_rows_equals_dense_rows(analytic):
+    """apply_rows gives the rows of A x for chosen fibers without forming A;
+    a dropped block, a transposed block or a wrong source index changes
+    them at O(1)."""
+    N, M, noqi = 9, 4, 3
+    pos, orient = _rand_layout(N, 3.0, 17)
+    A = oracle.assemble_matrix(analytic, EPS, pos, orient, M, noqi)
+    x = np.random.default_rng(3).normal(size=3 * M * N)
+    targets = np.array([0, 4, 8, 2])
+    y = oracle.apply_rows(analytic, EPS, pos, orient, M, noqi, x, targets)
+    m = 3 * M
+    want = np.concatenate([(A @ x)[t * m:(t + 1) * m] for t in targets])
+    np.testing.assert_allclose(y, want, rtol=0, atol=1e-13 * np.abs(want).max())
+    # blocks transposed in place would not do
+    At = A.copy()
+    for i in range(N):
+        for j in range(N):
+            At[i * m:(i + 1) * m, j * m:(j + 1) * m] = A[i * m:(i + 1) * m, j * m:(j + 1) * m].T
+    wrong = np.concatenate([(At @ x)[t * m:(t + 1) * m] for t in targets])
+    assert np.abs(wrong - y).max() > 1e-3
+
+
 def _rot(seed):
     q, _ = np.linalg.qr(np.random.default_rng(seed).normal(size=(3, 3)))
     if np.linalg.det(q) < 0:

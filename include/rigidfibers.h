@@ -137,7 +137,17 @@ typedef struct rf_stats {
     int64_t lu_panel_fallbacks;     /* LU panels redone with the pivot search over whole columns */
     int64_t operator_launches;      /* matrix-free operator evaluations (profile: timed) */
     double operator_ms_total;
-    int32_t last_refinements;       /* mixed mode: FP64 residuals of the last solve's refinement */
+    int32_t last_refinements;       /* mixed mode: FP64 residuals (matrix-free operator) of the last solve */
+    int32_t refine_inner_iters[8];  /* mixed mode: inner iterations (FP32 matrix) of the first 8 outer iterations */
+    int32_t last_outer_iters;       /* mixed mode: outer flexible-GMRES iterations on the FP64 operator */
+    double refine_history[8];       /* mixed mode: [0] FP64 backward error ||b - A x0|| / ||b|| of the initial
+                                       guess, [i] the outer Arnoldi estimate after outer iteration i */
+    double last_operator_noise;     /* mixed mode: rounding noise of the FP64 operator measured in the last
+                                       solve, ||A x - A x1 - A x2|| / ||b|| for x = x1 + x2 (0: not measured;
+                                       measured only when the outer Arnoldi estimate met gmres_tol and the
+                                       FP64 residual did not) */
+    int32_t last_noise_limited;     /* mixed mode: 1 when the last solve was accepted at the operator's noise
+                                       floor (backward error above gmres_tol, at most 16x the noise) */
     int32_t reserved_i;
     double reserved[2];
 } rf_stats;

@@ -99,12 +99,23 @@ class rf_stats(ctypes.Structure):
         ("operator_launches", ctypes.c_int64),
         ("operator_ms_total", ctypes.c_double),
         ("last_refinements", ctypes.c_int32),
+        ("refine_inner_iters", ctypes.c_int32 * 8),
+        ("last_outer_iters", ctypes.c_int32),
+        ("refine_history", ctypes.c_double * 8),
+        ("last_operator_noise", ctypes.c_double),
+        ("last_noise_limited", ctypes.c_int32),
         ("reserved_i", ctypes.c_int32),
         ("reserved", ctypes.c_double * 2),
     ]
 
     def as_dict(self):
-        return {f[0]: getattr(self, f[0]) for f in self._fields_ if not f[0].startswith("reserved")}
+        out = {}
+        for f in self._fields_:
+            if f[0].startswith("reserved"):
+                continue
+            v = getattr(self, f[0])
+            out[f[0]] = list(v) if isinstance(v, ctypes.Array) else v
+        return out
 
 
 _dp = ctypes.POINTER(ctypes.c_double)

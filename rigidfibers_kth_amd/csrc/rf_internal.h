@@ -128,6 +128,8 @@ hipError_t launch_advance(int N, int first, double dt, const double* x_prev, con
 hipError_t launch_fill(double* p, double v, size_t n, hipStream_t s);
 hipError_t launch_axpby(int n, double a, const double* x, double b, const double* y, double* out,
                         hipStream_t s);
+// x1 = u x, x2 = x - x1 with fixed pseudo-random u_i in [0, 1)
+hipError_t launch_random_split(int n, const double* x, double* x1, double* x2, hipStream_t s);
 
 // Thread-local error message.
 void set_error(const std::string& msg);

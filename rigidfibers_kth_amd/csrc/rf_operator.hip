@@ -50,23 +50,23 @@ struct OpKArgs {
 // Source vectors of the operator.  a_{j,0} (force term): mode 0 none; modes
 // 1, 2 the right-hand side's F/2 (only F_z with analytic integrals,
 // fortran/matrix_assembly.f90:857-859); mode 3 F/2.  a_{j,l}, l >= 1: the
-// coefficients x (none in mode 2).
-__global__ void op_source_kernel(int N, int M, int LQ, int analytic, int mode, const double* __restrict__ force,
-                                 const double* __restrict__ x, const double* __restrict__ orient,
-                                 const double* __restrict__ wp, double* __restrict__ src, int stride)
+// coefficients x (none in mode 2).  The host turns the mode into the masks
+// fmask (force components on) and x_on.
+__global__ void op_source_kernel(int N, int M, int LQ, int analytic, int fmask, int x_on,
+                                 const double* __restrict__ force, const double* __restrict__ x,
+                                 const double* __restrict__ orient, const double* __restrict__ wp,
+                                 double* __restrict__ src, int stride)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = analytic ? (M + 1) : LQ;
     if (idx >= N * per) return;
     const int j = idx / per, r = idx - j * per;
     double a[kMaxTerms + 1][3];
-    const bool f_on = (mode != 0);
-    const bool z_only = (mode == 1 || mode == 2) && analytic;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a[0][c] = (f_on && (!z_only || c == 2)) ? 0.5 * force[3 * j + c] : 0.0;
+    for (int c = 0; c < 3; ++c) a[0][c] = ((fmask >> c) & 1) ? 0.5 * force[3 * j + c] : 0.0;
     for (int l = 1; l <= M; ++l)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) a[l][c] = (mode == 2) ? 0.0 : x[(size_t)j * 3 * M + (l - 1) * 3 + c];
+        for (int c = 0; c < 3; ++c) a[l][c] = x_on ? x[(size_t)j * 3 * M + (l - 1) * 3 + c] : 0.0;
     double v[3] = {0.0, 0.0, 0.0};
     if (analytic) {
         // monomial coefficient m = r of f_j(s) = sum_l a_l P_l(s)
@@ -187,6 +187,8 @@ __global__ void __launch_bounds__(512) operator_kernel(const OpKArgs a)
 // for k >= 2 (fortran/matrix_assembly.f90:341-395).
 struct OpEpiArgs {
     int N, M, splits, mode;
+    int use_x;    // modes 0, 1: the identity term x
+    double sign;  // +1 mode 0, -1 modes 1, 2
     double D1;
     double Ek[kMaxTerms + 1], gam[kMaxTerms + 1];
     const double* orient;
@@ -220,17 +222,14 @@ __global__ void operator_epilogue_kernel(const OpEpiArgs a)
     const double td = t0 * s[0] + t1 * s[1] + t2 * s[2];
     const double tt[3] = {t0, t1, t2};
     const size_t row = (size_t)i * 3 * a.M + 3 * (k - 1);
+    // One arithmetic form for modes 0..2 (y = sign (x + P) with x = 0 in
+    // mode 2): a three-way branch on the mode here was compiled to a store
+    // of an undefined register for mode 2.
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const double pc = (k == 1) ? a.D1 * td * tt[c] : a.gam[k] * (s[c] - a.Ek[k] * td * tt[c]);
-        double v;
-        if (a.mode == 0)
-            v = a.x[row + c] + pc;
-        else if (a.mode == 1)
-            v = -(a.x[row + c] + pc);
-        else
-            v = -pc;
-        a.y[row + c] = v;
+        const double xv = a.use_x ? a.x[row + c] : 0.0;
+        a.y[row + c] = a.sign * (xv + pc);
     }
 }
 
@@ -290,8 +289,11 @@ hipError_t launch_operator(const OperatorPlan& p, const Tables& t, int mode, con
     {
         const int per = p.analytic ? (p.M + 1) : p.LQ;
         const int tot = p.N * per;
+        // force term: none for A x; F_z only for the analytic right-hand side
+        const int fmask = (mode == 0) ? 0 : ((mode == 1 || mode == 2) && p.analytic) ? 4 : 7;
+        const int x_on = (mode == 2) ? 0 : 1;
         hipLaunchKernelGGL(op_source_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, p.N, p.M, p.LQ, p.analytic,
-                           mode, force, x, orient, wp, src, p.src_stride);
+                           fmask, x_on, force, x, orient, wp, src, p.src_stride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -327,6 +329,8 @@ hipError_t launch_operator(const OperatorPlan& p, const Tables& t, int mode, con
     ea.M = p.M;
     ea.splits = p.splits;
     ea.mode = mode;
+    ea.use_x = (mode == 0 || mode == 1) ? 1 : 0;
+    ea.sign = (mode == 0) ? 1.0 : -1.0;
     ea.D1 = t.D1;
     for (int i = 0; i <= kMaxTerms; ++i) {
         ea.Ek[i] = t.Ek[i];

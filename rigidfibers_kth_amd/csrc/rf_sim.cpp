@@ -180,6 +180,12 @@ int rf_sim::alloc_all()
     ALLOC(d_op, operator_scratch_doubles(planOp));
     ALLOC(d_res64, (size_t)lda);
     ALLOC(d_dx, (size_t)lda);
+    if (mixed) {
+        outer_m = std::min(kOuterRestart, m);
+        ALLOC(d_fV, (size_t)lda * (outer_m + 1));
+        ALLOC(d_fZ, (size_t)lda * outer_m);
+        ALLOC(d_fw, (size_t)lda);
+    }
 #undef ALLOC
     HIP_TRY(hipMemsetAsync(d_counter, 0, sizeof(unsigned int), stream));
     // zero whole buffers once so the padding rows of A and W read as 0
@@ -934,63 +940,175 @@ int rf_sim::solve_system(int maxit, double tol, int* iters_out, double* be_out)
     return RF_OK;
 }
 
-// Mixed precision: iterative refinement of the FP32 matrix's solves against
-// the FP64 residual of the matrix-free operator,
-//     r = b - A x (FP64, no stored matrix);  solve A32 d = r to mixed_inner_tol;  x += d,
-// until ||r|| / ||b|| <= gmres_tol.  Each inner solve uses the configured
-// method on the FP32 matrix (GMRES with FP64 Arnoldi, its preconditioner and
-// FP32 LU factors, or the direct solver).
+// Rounding noise of the matrix-free FP64 operator at the current
+// configuration: ||A x - A x1 - A x2|| / bn with x = d_x split into two
+// pseudo-random summands (exact arithmetic gives 0).  Where two fibers come
+// within ~1e-3 of each other's axis the analytic moments reach ~1e12 and the
+// FP64 evaluation of those rows carries rounding errors far above 1e-16 ||b||.
+// Uses d_fw, d_fZ[0], d_dx and d_res64 as scratch.
+int rf_sim::operator_noise(double bn, double* eta)
+{
+    int rc;
+    double* x1 = d_fZ;
+    double* x2 = d_dx;
+    double* y = d_fw;
+    HIP_TRY(launch_random_split(n, d_x, x1, x2, stream));
+    if ((rc = apply_operator(kOpAx, d_x, y))) return rc;
+    if ((rc = apply_operator(kOpAx, x1, d_res64))) return rc;
+    HIP_TRY(launch_axpby(n, 1.0, y, -1.0, d_res64, y, stream));
+    if ((rc = apply_operator(kOpAx, x2, d_res64))) return rc;
+    HIP_TRY(launch_axpby(n, 1.0, y, -1.0, d_res64, y, stream));
+    double e2;
+    if ((rc = norm2(y, &e2))) return rc;
+    *eta = std::sqrt(e2) / bn;
+    return RF_OK;
+}
+
+// Mixed precision: flexible GMRES on the FP64 system, with the FP32 matrix's
+// solves as the preconditioner (GMRES-based iterative refinement):
+//     r = b - A x with the matrix-free FP64 operator (no stored FP64 matrix),
+//     v_0 = r / ||r||;  for j = 0, 1, ...:
+//         z_j = (solve A32 z = v_j to mixed_inner_tol with the configured method
+//                on the FP32 matrix: GMRES with FP64 Arnoldi, its preconditioner
+//                and FP32 LU factors, or the direct solver),
+//         w = A z_j (FP64 operator), orthogonalised against v_0..v_j (classical
+//         Gram-Schmidt applied twice), Givens rotations on the host;
+//     x += Z y once |g_{j+1}| / ||b|| <= gmres_tol, then the true FP64 residual.
+// Plain refinement (x += z_0, repeated) contracts by the spectral radius of
+// I - A A32^-1, which reaches ~1 for a few modes of the dense N=4000 systems
+// (FP32 storage and FP32 far-field recurrences leave A32 2e-7 from A, and
+// those systems are ill-conditioned enough); the Krylov outer iteration
+// removes such modes in a few steps at the same cost per step (one operator
+// evaluation and one inner solve).  When a restart cycle's Arnoldi estimate
+// meets the tolerance and the FP64 residual does not, the operator's own
+// rounding noise is measured once (operator_noise); a backward error within
+// kNoiseFactor of it is accepted and reported (last_noise_limited): no
+// FP64 evaluation of the system can resolve a smaller residual there.
 int rf_sim::solve_refined(int* iters_out, double* be_out)
 {
-    constexpr int kMaxRefinements = 30;
+    const int mo = outer_m;
     const double tol = p.gmres_tol;
     const double inner_tol = (p.mixed_inner_tol > 0.0) ? p.mixed_inner_tol : 1e-6;
     const int maxit = p.gmres_max_iters;
     int rc;
     double bn2;
-    if ((rc = norm2(d_b, &bn2))) return rc;  // the scale only: FP32-accurate is enough
+    if ((rc = norm2(d_b, &bn2))) return rc;  // the scale only
     const double bn = std::sqrt(bn2);
     *iters_out = 0;
     stats.last_refinements = 0;
+    stats.last_outer_iters = 0;
+    stats.last_operator_noise = 0.0;
+    stats.last_noise_limited = 0;
+    for (int i = 0; i < 8; ++i) {
+        stats.refine_history[i] = 0.0;
+        stats.refine_inner_iters[i] = 0;
+    }
     if (bn == 0.0) {
         HIP_TRY(launch_fill(d_x, 0.0, (size_t)n, stream));
         *be_out = 0.0;
         return RF_OK;
     }
     r0_ready = false;  // a residual of the learned guess is on the FP32 matrix
-    int total = 0;
-    double be = 0.0, be_prev = 0.0;
-    int stalls = 0;
-    for (int k = 0;; ++k) {
-        if ((rc = apply_operator(kOpResidual, d_x, d_res64))) return rc;
+    const int ldh = mo + 1;
+    std::vector<double> H((size_t)(mo + 1) * (mo + 1)), g(mo + 1), cs(mo + 1), sn(mo + 1), y(mo + 1);
+    const int m = restart;  // slot layout of d_dots (inner GMRES)
+    double* D1 = d_dots;
+    double* D2 = d_dots + (m + 2);
+    double* D3 = d_dots + 2 * (m + 2);
+    int total = 0, outer = 0;
+    double be = 0.0;
+    // the FP64 residual b - A x; ||r||^2 stays in d_dots[0] for the scaling
+    auto residual = [&](double* beta) -> int {
+        int r = apply_operator(kOpResidual, d_x, d_res64);
+        if (r) return r;
         double rr;
-        if ((rc = norm2(d_res64, &rr))) return rc;
-        be = std::sqrt(rr) / bn;
-        stats.last_refinements = k + 1;
-        if (k == 0) stats.last_initial_residual = be;
-        if (be <= tol) break;
-        if (k > 0 && be > 0.5 * be_prev) ++stalls;
-        if (k >= kMaxRefinements || total >= maxit || stalls >= 3) {
-            rf::set_error("mixed precision: refinement against the FP64 operator stalled at backward error " +
-                          std::to_string(be) + " after " + std::to_string(k) + " corrections");
+        if ((r = norm2(d_res64, &rr))) return r;
+        *beta = std::sqrt(rr);
+        return RF_OK;
+    };
+    double beta;
+    if ((rc = residual(&beta))) return rc;
+    be = beta / bn;
+    stats.refine_history[0] = be;
+    stats.last_initial_residual = be;
+    stats.last_refinements = 1;
+    while (be > tol) {
+        if (outer >= kMaxOuterIters || total >= maxit) {
+            char msg[200];
+            snprintf(msg, sizeof(msg),
+                     "mixed precision: flexible GMRES on the FP64 operator reached backward error %.3e after %d "
+                     "outer and %d inner iterations (tolerance %.1e)", be, outer, total, tol);
+            rf::set_error(msg);
+            stats.last_outer_iters = outer;
+            stats.last_backward_error = be;
             *iters_out = total;
             *be_out = be;
             return RF_ENOCONV;
         }
-        be_prev = be;
-        // inner solve A32 d = r from d = 0 on the buffers of the plain solve
-        std::swap(d_b, d_res64);
-        std::swap(d_x, d_dx);
-        int it = 0;
-        double be_in = 0.0;
-        rc = launch_fill(d_x, 0.0, (size_t)n, stream) == hipSuccess ? RF_OK : RF_EHIP;
-        if (rc == RF_OK) rc = solve_system(std::max(1, maxit - total), inner_tol, &it, &be_in);
-        std::swap(d_b, d_res64);
-        std::swap(d_x, d_dx);
-        if (rc) return rc;
-        total += it;
-        HIP_TRY(launch_axpby(n, 1.0, d_x, 1.0, d_dx, d_x, stream));
+        HIP_TRY(launch_scale_copy(d_res64, d_dots, 0, n, d_fV, stream));
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        double bea = 1.0;
+        for (int j = 0; j < mo; ++j) {
+            // z_j = A32^-1 v_j on the buffers of the plain solve
+            HIP_TRY(launch_copy_vec(d_fV + (size_t)j * lda, d_res64, n, stream));
+            std::swap(d_b, d_res64);
+            std::swap(d_x, d_dx);
+            int it = 0;
+            double be_in = 0.0;
+            rc = launch_fill(d_x, 0.0, (size_t)n, stream) == hipSuccess ? RF_OK : RF_EHIP;
+            if (rc == RF_OK) rc = solve_system(std::max(1, maxit - total), inner_tol, &it, &be_in);
+            std::swap(d_b, d_res64);
+            std::swap(d_x, d_dx);
+            if (rc) return rc;
+            if (outer < 8) stats.refine_inner_iters[outer] = it;
+            total += it;
+            double* zj = d_fZ + (size_t)j * lda;
+            HIP_TRY(launch_copy_vec(d_dx, zj, n, stream));
+            if ((rc = apply_operator(kOpAx, zj, d_fw))) return rc;
+            k = j + 1;
+            HIP_TRY(launch_multidot(d_fV, lda, k, d_fw, n, d_dparts, d_counter, D1, stream));
+            HIP_TRY(launch_orth_dots(d_fV, lda, k, D1, d_fw, n, 1, d_dparts, d_counter, D2, stream));
+            HIP_TRY(launch_orth_dots(d_fV, lda, k, D2, d_fw, n, 0, d_dparts, d_counter, D3, stream));
+            if (k < mo) HIP_TRY(launch_scale_copy(d_fw, D3, 0, n, d_fV + (size_t)k * lda, stream));
+            if ((rc = fetch_dots(3 * (m + 2)))) return rc;
+            for (int i = 0; i < k; ++i) H[i + (size_t)j * ldh] = h_dots[i] + h_dots[(m + 2) + i];
+            H[k + (size_t)j * ldh] = std::sqrt(h_dots[2 * (m + 2)]);
+            for (int i = 0; i < j; ++i) rot1(&H[i + (size_t)j * ldh], &H[i + 1 + (size_t)j * ldh], cs[i], sn[i]);
+            givens(H[j + (size_t)j * ldh], H[k + (size_t)j * ldh], &cs[j], &sn[j]);
+            rot1(&g[j], &g[k], cs[j], sn[j]);
+            rot1(&H[j + (size_t)j * ldh], &H[k + (size_t)j * ldh], cs[j], sn[j]);
+            H[k + (size_t)j * ldh] = 0.0;
+            ++outer;
+            bea = std::fabs(g[k]) / bn;
+            if (outer < 8) stats.refine_history[outer] = bea;
+            if (bea <= tol || outer >= kMaxOuterIters || total >= maxit) break;
+        }
+        // x += Z y, then the true FP64 residual
+        hess_solve(H, ldh, g, k, y);
+        std::copy(y.begin(), y.begin() + k, h_y);
+        HIP_TRY(launch_vy_plus_x(d_fZ, lda, k, h_y, d_x, n, d_xc, stream));
+        HIP_TRY(launch_copy_vec(d_xc, d_x, n, stream));
+        if ((rc = residual(&beta))) return rc;
+        be = beta / bn;
+        stats.last_refinements += 1;
+        if (be > tol && bea <= tol) {
+            // the Krylov model converged, the FP64 residual did not: noise?
+            if (stats.last_operator_noise == 0.0) {
+                double eta = 0.0;
+                if ((rc = operator_noise(bn, &eta))) return rc;
+                stats.last_operator_noise = eta;
+                if ((rc = residual(&beta))) return rc;  // d_res64 and d_dots[0] for the next cycle
+            }
+            if (be <= kNoiseFactor * stats.last_operator_noise) {
+                stats.last_noise_limited = 1;
+                break;
+            }
+        }
     }
+    stats.last_outer_iters = outer;
     *iters_out = total;
     *be_out = be;
     return RF_OK;

@@ -63,6 +63,16 @@ struct rf_sim {
     double* d_dparts = nullptr;
     double* d_dots = nullptr;
     unsigned int* d_counter = nullptr;
+    // mixed mode: basis and preconditioned directions of the outer flexible
+    // GMRES on the FP64 operator
+    static constexpr int kOuterRestart = 10;   // outer basis vectors (at most the inner restart)
+    static constexpr int kMaxOuterIters = 40;  // outer iterations per solve
+    static constexpr double kNoiseFactor = 16.0; // accepted backward error / measured operator noise
+    int operator_noise(double bn, double* eta);
+    int outer_m = 0;
+    double* d_fV = nullptr;  // [outer_m + 1][lda]
+    double* d_fZ = nullptr;  // [outer_m][lda]
+    double* d_fw = nullptr;  // [lda]
     double* h_dots = nullptr;  // pinned
     double* h_y = nullptr;     // pinned
     // neighbourhoods and preconditioner (allocated on first use)

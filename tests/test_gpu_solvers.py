@@ -149,7 +149,8 @@ def test_lu_preconditioned_gmres(precision):
     pos, ori = _hard_state()
     out = {}
     for pc in (rf.PRECOND_NONE, rf.PRECOND_LU):
-        with _sim(pos, ori, precond=pc, precision=precision) as sim:
+        # unpreconditioned inner solves take ~1000 iterations each on this state
+        with _sim(pos, ori, precond=pc, precision=precision, gmres_max_iters=20000) as sim:
             sim.assemble_system()
             sim.solve_system()
             st = sim.stats()

@@ -267,6 +267,20 @@ int rf_matvec(rf_sim *sim, const double *x, double *y);
 #define RF_OP_VELOCITY 3
 int rf_operator(rf_sim *sim, int32_t mode, const double *x, double *y);
 
+/* rf_operator on caller-provided device buffers, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  pos, orient, force (3N doubles each, the per-fiber
+ * external force), x and y (n doubles; 6N for RF_OP_VELOCITY: V then Omega)
+ * are device pointers; the work is queued on `stream` (a hipStream_t, NULL =
+ * the default stream) and the call returns without waiting.  Scratch is one
+ * 32 MiB device array per GPU, held until the process ends; a system that
+ * does not fit (roughly N > 40000 with M = 5) is rejected with RF_EINVAL.
+ * All calls on one GPU share that array: calls that overlap on one GPU (other
+ * host threads, other streams) race. */
+int rf_operator_noalloc(const rf_params *params, int32_t num_fibers, int32_t mode, const double *pos,
+                        const double *orient, const double *force, const double *x, double *y, void *stream);
+
 /* Restart.  Everything the next step reads, as the two-level scheme of
  * fortran/advect_fibers.f90:498-523 needs it: the current positions and
  * orientations (X^n, t^n), those of the previous level (X^{n-1}, t^{n-1}),

@@ -159,6 +159,9 @@ _SIGNATURES = {
     "rf_set_coefficients": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "rf_matvec": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "rf_operator": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, _dp]),
+    "rf_operator_noalloc": (ctypes.c_int, [ctypes.POINTER(rf_params), ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p]),
     "rf_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "rf_get_neighbours": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _dp]),
     "rf_precond_apply": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),

@@ -30,8 +30,11 @@
 // R = X - x_j - s_q t_j, f_q = w_q f_j(s_q).
 #include "rf_inner.h"
 #include "rf_internal.h"
+#include "rigidfibers.h"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace rf {
 namespace {
@@ -345,4 +348,143 @@ hipError_t launch_operator(const OperatorPlan& p, const Tables& t, int mode, con
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// The operator on caller-provided device buffers without any allocation
+// (rf_operator_noalloc).  Its scratch -- the quadrature tables, the
+// velocity sums and the operator's source vectors and split partials -- is
+// one file-scope device array per GPU, resolved once per device with
+// hipGetSymbolAddress; host code uses only that address.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 22;  // 32 MiB per GPU
+constexpr size_t kTableDoubles = 1024;                      // wp[(M+1) LQ] (<= 864), pv[LQ] at 864
+constexpr size_t kPvOffset = (kMaxTerms + 1) * kMaxQuad;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// up to 128 table values per launch, passed by value
+struct TableChunk {
+    int offset, count;
+    double v[128];
+};
+
+__global__ void store_table_kernel(double* __restrict__ dst, const TableChunk c)
+{
+    const int i = threadIdx.x;
+    if (i < c.count) dst[c.offset + i] = c.v[i];
+}
+
+hipError_t store_tables(double* dst, const Tables& t, hipStream_t s)
+{
+    const int nwp = (t.M + 1) * t.LQ;
+    auto put = [&](const double* src, int count, int offset) -> hipError_t {
+        for (int b = 0; b < count; b += 128) {
+            TableChunk c;
+            c.offset = offset + b;
+            c.count = std::min(128, count - b);
+            for (int i = 0; i < c.count; ++i) c.v[i] = src[b + i];
+            hipLaunchKernelGGL(store_table_kernel, dim3(1), dim3(128), 0, s, dst, c);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    hipError_t e = put(t.wp, nwp, 0);
+    if (e == hipSuccess) e = put(t.pv, t.LQ, (int)kPvOffset);
+    return e;
+}
+
+}  // namespace
+
+// The device address of this GPU's scratch array (set-up once per device).
+static int noalloc_scratch(double** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("rf_operator_noalloc: no current HIP device");
+        return RF_EHIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("rf_operator_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return RF_EHIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return RF_OK;
+}
+
 }  // namespace rf
+
+extern "C" int rf_operator_noalloc(const rf_params* params, int32_t num_fibers, int32_t mode, const double* pos,
+                                   const double* orient, const double* force, const double* x, double* y,
+                                   void* stream)
+{
+    using namespace rf;
+    if (!params || !pos || !orient || !force || !y || (mode != RF_OP_RHS && !x) || mode < RF_OP_AX ||
+        mode > RF_OP_VELOCITY) {
+        set_error("rf_operator_noalloc: null buffer or unknown mode");
+        return RF_EINVAL;
+    }
+    const rf_params& P = *params;
+    if (num_fibers < 2 || P.num_terms < 1 || P.num_terms > kMaxTerms || P.num_quad_intervals < 1 ||
+        3 * P.num_quad_intervals > kMaxQuad || !(P.slenderness > 0.0)) {
+        set_error("rf_operator_noalloc: invalid parameters (need N>=2, 1<=M<=8, 1<=NoQI<=32, eps>0)");
+        return RF_EINVAL;
+    }
+    Tables tab;
+    build_tables(P.num_terms, P.num_quad_intervals, P.slenderness, &tab);
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        set_error("rf_operator_noalloc: cannot query the device");
+        return RF_EHIP;
+    }
+    const int N = num_fibers;
+    OperatorPlan plan = plan_operator(N, tab.M, tab.LQ, P.use_analytical_integration ? 1 : 0, cus);
+    // fewer source splits where the partials would not fit
+    const size_t fixed = kTableDoubles + 6 * (size_t)N + (size_t)N * plan.src_stride;
+    const size_t per_split = (size_t)N * (tab.M + 1) * 3;
+    if (fixed + per_split > kNoAllocScratchDoubles) {
+        set_error("rf_operator_noalloc: " + std::to_string(N) + " fibers need " +
+                  std::to_string((fixed + per_split) * sizeof(double)) + " bytes of scratch; the maximum is " +
+                  std::to_string(kNoAllocScratchDoubles * sizeof(double)) + " (rf_operator has no limit)");
+        return RF_EINVAL;
+    }
+    const size_t max_splits = (kNoAllocScratchDoubles - fixed) / per_split;
+    if ((size_t)plan.splits > max_splits) {
+        plan.splits = (int)max_splits;
+        plan.src_per_split = (N + plan.splits - 1) / plan.splits;
+        plan.splits = (N + plan.src_per_split - 1) / plan.src_per_split;
+    }
+    double* base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    double* wp = base;
+    double* pv = base + kPvOffset;
+    double* y6 = base + kTableDoubles;
+    double* work = y6 + 6 * (size_t)N;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = store_tables(base, tab, s);
+    const int kmode = (mode == RF_OP_AX) ? kOpAx : (mode == RF_OP_RESIDUAL) ? kOpResidual
+                                                : (mode == RF_OP_RHS) ? kOpRhs : kOpVelocity;
+    if (e == hipSuccess)
+        e = launch_operator(plan, tab, kmode, pos, orient, force, x, wp, pv, work, (mode == RF_OP_VELOCITY) ? y6 : y,
+                            s);
+    if (e == hipSuccess && mode == RF_OP_VELOCITY)
+        e = launch_velocity_epilogue(N, tab.d, tab.e, pos, orient, force, y6, y, y + 3 * (size_t)N, s);
+    if (e != hipSuccess) {
+        set_error(std::string("rf_operator_noalloc: ") + hipGetErrorString(e));
+        return RF_EHIP;
+    }
+    return RF_OK;
+}

@@ -106,6 +106,9 @@ typedef struct rf_params {
                                      take trailing updates on fp16-rounded operands (FP32 accumulation);
                                      the direct solver's factors keep FP32 updates */
     double refine_tol;            /* GMRES tolerance once the LU preconditioner is in use (0 = gmres_tol) */
+    double shear_rate;            /* background shear flow (shear_rate * y, 0, 0) added to the velocities: the
+                                     reference's `sh`, hard-coded 0 there (fortran/coeffs_and_vels.f90:574,
+                                     597-600); like the reference it enters the velocities only */
 } rf_params;
 
 typedef struct rf_stats {
@@ -162,7 +165,8 @@ void rf_params_default(rf_params *p);
  * Lines after the last mandatory one may hold "key value" extensions:
  * precision fp64|mixed, warm_start prev|zero|extrap|learned, profile 0|1,
  * lu_block NB, precond none|neighbour|lu|auto, precond_neighbours K, precond_switch_iters I,
- * lu_factor_fp32 0|1, lu_column_panel 0|1, lu_update_fp16 0|1, refine_tol T, mixed_inner_tol T. */
+ * lu_factor_fp32 0|1, lu_column_panel 0|1, lu_update_fp16 0|1, refine_tol T, mixed_inner_tol T,
+ * shear_rate S. */
 int rf_parse_parameters(const char *path, rf_params *out);
 
 /* XcT_init layout (src/parameters.cc:186-227): N, then x y z and tx ty tz per

@@ -51,14 +51,15 @@ def lib():
         L.orc_mat_prod.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                    ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_compute_velocities.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                             _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+                                             _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                             ctypes.c_double]
         L.orc_assemble_rhs_rows.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                             _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, _dp]
         L.orc_apply_rows.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                      ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, _ip, _dp]
         L.orc_compute_velocities_rows.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                   _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp,
-                                                  ctypes.c_int, _ip, _dp, _dp]
+                                                  ctypes.c_int, _ip, _dp, _dp, ctypes.c_double]
         L.orc_lu_solve.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_lu_solve.restype = ctypes.c_int
         L.orc_gmres.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
@@ -67,7 +68,7 @@ def lib():
         L.orc_simulate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip, ctypes.c_double]
         L.orc_simulate.restype = ctypes.c_int
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
@@ -162,14 +163,14 @@ def mat_prod(eps, pos, orient, M, noqi, x):
     return y
 
 
-def compute_velocities(analytic, eps, pos, orient, force, M, noqi, coeff):
+def compute_velocities(analytic, eps, pos, orient, force, M, noqi, coeff, sh=0.0):
     pos, orient, force, coeff = (_f64(a).ravel() for a in (pos, orient, force, coeff))
     N = pos.size // 3
     pv, wv, lv = _setup(M, noqi)
     vel = np.empty(3 * N)
     rot = np.empty(3 * N)
     lib().orc_compute_velocities(int(analytic), eps, N, M, _p(pos), _p(orient), _p(force), pv.size, _p(pv),
-                                 _p(wv), _p(lv), _p(coeff), _p(vel), _p(rot))
+                                 _p(wv), _p(lv), _p(coeff), _p(vel), _p(rot), float(sh))
     return vel, rot
 
 
@@ -196,7 +197,7 @@ def apply_rows(analytic, eps, pos, orient, M, noqi, x, targets):
     return y
 
 
-def compute_velocities_rows(analytic, eps, pos, orient, force, M, noqi, coeff, targets):
+def compute_velocities_rows(analytic, eps, pos, orient, force, M, noqi, coeff, targets, sh=0.0):
     pos, orient, force, coeff = (_f64(a).ravel() for a in (pos, orient, force, coeff))
     N = pos.size // 3
     pv, wv, lv = _setup(M, noqi)
@@ -204,7 +205,8 @@ def compute_velocities_rows(analytic, eps, pos, orient, force, M, noqi, coeff, t
     vel = np.empty(3 * t.size)
     rot = np.empty(3 * t.size)
     lib().orc_compute_velocities_rows(int(analytic), eps, N, M, _p(pos), _p(orient), _p(force), pv.size, _p(pv),
-                                      _p(wv), _p(lv), _p(coeff), t.size, t.ctypes.data_as(_ip), _p(vel), _p(rot))
+                                      _p(wv), _p(lv), _p(coeff), t.size, t.ctypes.data_as(_ip), _p(vel), _p(rot),
+                                      float(sh))
     return vel, rot
 
 
@@ -238,7 +240,7 @@ def gmres(A, b, x0, restart, max_iters, tol):
 
 
 def simulate(analytic, direct, eps, dt, steps, pos0, orient0, M=5, noqi=8, restart=10, max_iters=100,
-             tol=1e-5, force=None):
+             tol=1e-5, force=None, sh=0.0):
     """Runs the reference time loop; returns dict of per-step arrays."""
     pos0, orient0 = _f64(pos0).ravel(), _f64(orient0).ravel()
     N = pos0.size // 3
@@ -252,7 +254,7 @@ def simulate(analytic, direct, eps, dt, steps, pos0, orient0, M=5, noqi=8, resta
     rc = lib().orc_simulate(int(analytic), int(direct), eps, dt, steps, N, M, noqi, restart, max_iters, tol,
                             _p(force), _p(pos0), _p(orient0), _p(out["b"]), _p(out["x"]), _p(out["vel"]),
                             _p(out["rot"]), _p(out["pos"]), _p(out["orient"]),
-                            iters.ctypes.data_as(_ip))
+                            iters.ctypes.data_as(_ip), float(sh))
     if rc:
         raise RuntimeError("oracle simulate failed rc=%d" % rc)
     out["iters"] = iters

@@ -626,17 +626,17 @@ void orc_mat_prod(double eps, int N, int M, const double *pos, const double *ori
 void orc_compute_velocities(int analytic, double eps, int N, int M,
                             const double *pos, const double *orient, const double *force,
                             int LQ, const double *pv, const double *wv, const double *lvec,
-                            const double *coeff, double *vel, double *rot)
+                            const double *coeff, double *vel, double *rot, double sh)
 {
     orc_compute_velocities_rows(analytic, eps, N, M, pos, orient, force, LQ, pv, wv, lvec, coeff,
-                                N, NULL, vel, rot);
+                                N, NULL, vel, rot, sh);
 }
 
 void orc_compute_velocities_rows(int analytic, double eps, int N, int M,
                                  const double *pos, const double *orient, const double *force,
                                  int LQ, const double *pv, const double *wv, const double *lvec,
                                  const double *coeff, int ntargets, const int *targets,
-                                 double *vel, double *rot)
+                                 double *vel, double *rot, double sh)
 {
     double c = log(eps * eps * exp(1.0));
     double d = -c, e = 2.0, mubar = d;
@@ -654,10 +654,11 @@ void orc_compute_velocities_rows(int analytic, double eps, int N, int M,
             const double *Fa = force + 3 * filno;
             double tdF = ta[0] * Fa[0] + ta[1] * Fa[1] + ta[2] * Fa[2];
             double V[3], R[3];
-            for (int a = 0; a < 3; ++a) {
-                V[a] = 0.5 * ((d + e) * Fa[a] + (d - e) * tdF * ta[a]);
-                R[a] = 0.0;
-            }
+            for (int a = 0; a < 3; ++a) V[a] = 0.5 * ((d + e) * Fa[a] + (d - e) * tdF * ta[a]);
+            /* background shear u = (sh y, 0, 0): fortran/coeffs_and_vels.f90:597-600 */
+            V[0] = V[0] + sh * mubar * xc[1];
+            for (int a = 0; a < 3; ++a) R[a] = -sh * mubar * ta[1] * ta[0] * ta[a];
+            R[0] = R[0] + sh * mubar * ta[1];
             for (int fno = 0; fno < N; ++fno) {
                 if (fno == filno) continue;
                 const double *xcb = pos + 3 * fno, *tb = orient + 3 * fno;
@@ -973,7 +974,7 @@ int orc_simulate(int analytic, int direct, double eps, double dt, int steps,
                  int N, int M, int noqi, int restart, int max_iters, double tol,
                  const double *force, const double *pos0, const double *orient0,
                  double *b_out, double *x_out, double *vel_out, double *rot_out,
-                 double *pos_out, double *orient_out, int *iters_out)
+                 double *pos_out, double *orient_out, int *iters_out, double sh)
 {
     int LQ = 3 * noqi, n = 3 * M * N, N3 = 3 * N;
     double *pv = (double *)malloc(sizeof(double) * LQ);
@@ -1028,7 +1029,7 @@ int orc_simulate(int analytic, int direct, double eps, double dt, int steps,
         oldv = newv;
         newv = 1 - oldv;
         orc_compute_velocities(analytic, eps, N, M, X[three], T[three], force, LQ, pv, wv, lvec, x,
-                               Vv[newv], Rv[newv]);
+                               Vv[newv], Rv[newv], sh);
         if (vel_out) memcpy(vel_out + (size_t)(tt - 1) * N3, Vv[newv], sizeof(double) * N3);
         if (rot_out) memcpy(rot_out + (size_t)(tt - 1) * N3, Rv[newv], sizeof(double) * N3);
         one = two;

@@ -69,18 +69,20 @@ void orc_mat_prod(double eps, int N, int M, const double *pos, const double *ori
                   int LQ, const double *pv, const double *wv, const double *lvec,
                   const double *x, double *y);
 
-/* Velocities. analytic=1: fortran/coeffs_and_vels.f90:647-746; 0: :543-644. */
+/* Velocities. analytic=1: fortran/coeffs_and_vels.f90:647-746; 0: :543-644.
+ * sh: rate of the background shear flow (sh y, 0, 0), the reference's `sh`
+ * (hard-coded 0 there, fortran/coeffs_and_vels.f90:574,597-600). */
 void orc_compute_velocities(int analytic, double eps, int N, int M,
                             const double *pos, const double *orient, const double *force,
                             int LQ, const double *pv, const double *wv, const double *lvec,
-                            const double *coeff, double *vel, double *rot);
+                            const double *coeff, double *vel, double *rot, double sh);
 
 /* V, Omega of the fibers targets[0..ntargets) only (vel, rot: 3*ntargets). */
 void orc_compute_velocities_rows(int analytic, double eps, int N, int M,
                                  const double *pos, const double *orient, const double *force,
                                  int LQ, const double *pv, const double *wv, const double *lvec,
                                  const double *coeff, int ntargets, const int *targets,
-                                 double *vel, double *rot);
+                                 double *vel, double *rot, double sh);
 
 /* LU with partial pivoting; overwrites A with factors and b with x.
  * Stands in for LAPACK dgesv (fortran/coeffs_and_vels.f90:62,158).
@@ -108,7 +110,7 @@ int orc_simulate(int analytic, int direct, double eps, double dt, int steps,
                  int N, int M, int noqi, int restart, int max_iters, double tol,
                  const double *force, const double *pos0, const double *orient0,
                  double *b_out, double *x_out, double *vel_out, double *rot_out,
-                 double *pos_out, double *orient_out, int *iters_out);
+                 double *pos_out, double *orient_out, int *iters_out, double sh);
 
 /* Threads OpenMP will use (for reporting the CPU baseline core count). */
 int orc_num_threads(void);

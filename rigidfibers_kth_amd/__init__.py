@@ -61,6 +61,7 @@ class rf_params(ctypes.Structure):
         ("lu_column_panel", ctypes.c_int32),
         ("lu_update_fp16", ctypes.c_int32),
         ("refine_tol", ctypes.c_double),
+        ("shear_rate", ctypes.c_double),
     ]
 
 

@@ -118,7 +118,7 @@ constexpr int kMaxDotBlocks = 64;
 
 // Velocity epilogue, advance (rf_step.hip).
 hipError_t launch_build_fext(int N, int M, const double* force, const double* x, double* fext, hipStream_t s);
-hipError_t launch_velocity_epilogue(int N, double d, double e, const double* pos, const double* orient,
+hipError_t launch_velocity_epilogue(int N, double d, double e, double sh, const double* pos, const double* orient,
                                     const double* force, const double* y6, double* vel, double* rot,
                                     hipStream_t s);
 hipError_t launch_advance(int N, int first, double dt, const double* x_prev, const double* x_cur,

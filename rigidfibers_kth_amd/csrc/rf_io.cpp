@@ -38,6 +38,7 @@ extern "C" void rf_params_default(rf_params* p)
     p->profile = 0;
     p->lu_block = 0;
     p->mixed_inner_tol = 0.0;
+    p->shear_rate = 0.0;  // fortran/coeffs_and_vels.f90:574
 }
 
 namespace {
@@ -179,6 +180,8 @@ extern "C" int rf_parse_parameters(const char* path, rf_params* out)
             ok = parse_double(val, &p.refine_tol);
         } else if (key == "mixed_inner_tol") {
             ok = parse_double(val, &p.mixed_inner_tol);
+        } else if (key == "shear_rate") {
+            ok = parse_double(val, &p.shear_rate);
         } else {
             ok = false;
         }

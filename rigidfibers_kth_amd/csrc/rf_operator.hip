@@ -481,7 +481,7 @@ extern "C" int rf_operator_noalloc(const rf_params* params, int32_t num_fibers, 
         e = launch_operator(plan, tab, kmode, pos, orient, force, x, wp, pv, work, (mode == RF_OP_VELOCITY) ? y6 : y,
                             s);
     if (e == hipSuccess && mode == RF_OP_VELOCITY)
-        e = launch_velocity_epilogue(N, tab.d, tab.e, pos, orient, force, y6, y, y + 3 * (size_t)N, s);
+        e = launch_velocity_epilogue(N, tab.d, tab.e, P.shear_rate, pos, orient, force, y6, y, y + 3 * (size_t)N, s);
     if (e != hipSuccess) {
         set_error(std::string("rf_operator_noalloc: ") + hipGetErrorString(e));
         return RF_EHIP;

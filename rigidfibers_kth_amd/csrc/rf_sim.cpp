@@ -1174,7 +1174,7 @@ int rf_sim::update_velocities()
         HIP_TRY(launch_gemv(planW, d_W, d_fext, d_gparts, d_y6, nullptr, 0, stream, ea, eb));
         if (p.profile) timed_push(ea, eb, kGemvW);
     }
-    HIP_TRY(launch_velocity_epilogue(N, tab.d, tab.e, d_pos[lvl_three], d_orient[lvl_three], d_force, d_y6,
+    HIP_TRY(launch_velocity_epilogue(N, tab.d, tab.e, p.shear_rate, d_pos[lvl_three], d_orient[lvl_three], d_force, d_y6,
                                      d_vel[vel_new], d_rot[vel_new], stream));
     stats.ms_velocities = phase_end();
     stats.total_ms_velocities += stats.ms_velocities;
@@ -1577,7 +1577,7 @@ extern "C" int rf_operator(rf_sim* s, int32_t mode, const double* x, double* y)
     // V and Omega of these coefficients (the handle's own velocities stay unchanged)
     double* v = s->d_r;
     double* w = s->d_r + 3 * (size_t)s->N;
-    HIP_TRY(launch_velocity_epilogue(s->N, s->tab.d, s->tab.e, s->d_pos[s->lvl_three], s->d_orient[s->lvl_three],
+    HIP_TRY(launch_velocity_epilogue(s->N, s->tab.d, s->tab.e, s->p.shear_rate, s->d_pos[s->lvl_three], s->d_orient[s->lvl_three],
                                      s->d_force, s->d_y6, v, w, s->stream));
     return copy_out(s, y, v, 6 * (size_t)s->N);
 }

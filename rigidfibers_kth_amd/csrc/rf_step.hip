@@ -19,11 +19,14 @@ __global__ void build_fext_kernel(int N, int M, const double* __restrict__ force
     fext[idx] = (l == 0) ? 0.5 * force[3 * j + c] : x[(size_t)j * 3 * M + (l - 1) * 3 + c];
 }
 
-// V_i = [1/2((d+e)F + (d-e)(t.F)t) + 1/2 y_V]/d,  Omega_i = 3/2 (y_O - (t.y_O) t)/d
-// (fortran/coeffs_and_vels.f90:596,619-627); y6[6i..6i+5] = (y_V, y_O).
-__global__ void velocity_epilogue_kernel(int N, double d, double e, const double* __restrict__ orient,
-                                         const double* __restrict__ force, const double* __restrict__ y6,
-                                         double* __restrict__ vel, double* __restrict__ rot)
+// V_i = [1/2((d+e)F + (d-e)(t.F)t) + sh d y_i e_x + 1/2 y_V]/d,
+// Omega_i = [sh d t_y (e_x - t_x t) + 3/2 (y_O - (t.y_O) t)]/d
+// (fortran/coeffs_and_vels.f90:596-600,619-627, mubar = d); y6[6i..6i+5] =
+// (y_V, y_O); sh is the rate of the background shear flow (sh y, 0, 0).
+__global__ void velocity_epilogue_kernel(int N, double d, double e, double sh, const double* __restrict__ pos,
+                                         const double* __restrict__ orient, const double* __restrict__ force,
+                                         const double* __restrict__ y6, double* __restrict__ vel,
+                                         double* __restrict__ rot)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -32,13 +35,15 @@ __global__ void velocity_epilogue_kernel(int N, double d, double e, const double
     const double tF = t0 * F0 + t1 * F1 + t2 * F2;
     const double* y = y6 + 6 * i;
     const double inv = 1.0 / d;
-    vel[3 * i + 0] = (0.5 * ((d + e) * F0 + (d - e) * tF * t0) + 0.5 * y[0]) * inv;
+    const double shd = sh * d;
+    vel[3 * i + 0] = (0.5 * ((d + e) * F0 + (d - e) * tF * t0) + shd * pos[3 * i + 1] + 0.5 * y[0]) * inv;
     vel[3 * i + 1] = (0.5 * ((d + e) * F1 + (d - e) * tF * t1) + 0.5 * y[1]) * inv;
     vel[3 * i + 2] = (0.5 * ((d + e) * F2 + (d - e) * tF * t2) + 0.5 * y[2]) * inv;
     const double ty = t0 * y[3] + t1 * y[4] + t2 * y[5];
-    rot[3 * i + 0] = 1.5 * (y[3] - ty * t0) * inv;
-    rot[3 * i + 1] = 1.5 * (y[4] - ty * t1) * inv;
-    rot[3 * i + 2] = 1.5 * (y[5] - ty * t2) * inv;
+    const double st = -shd * t1 * t0;
+    rot[3 * i + 0] = (st * t0 + shd * t1 + 1.5 * (y[3] - ty * t0)) * inv;
+    rot[3 * i + 1] = (st * t1 + 1.5 * (y[4] - ty * t1)) * inv;
+    rot[3 * i + 2] = (st * t2 + 1.5 * (y[5] - ty * t2)) * inv;
 }
 
 // Euler first step / 2nd-order multistep + normalisation
@@ -113,11 +118,11 @@ hipError_t launch_build_fext(int N, int M, const double* force, const double* x,
     return hipGetLastError();
 }
 
-hipError_t launch_velocity_epilogue(int N, double d, double e, const double* /*pos*/, const double* orient,
+hipError_t launch_velocity_epilogue(int N, double d, double e, double sh, const double* pos, const double* orient,
                                     const double* force, const double* y6, double* vel, double* rot, hipStream_t s)
 {
-    hipLaunchKernelGGL(velocity_epilogue_kernel, dim3((N + 127) / 128), dim3(128), 0, s, N, d, e, orient, force, y6,
-                       vel, rot);
+    hipLaunchKernelGGL(velocity_epilogue_kernel, dim3((N + 127) / 128), dim3(128), 0, s, N, d, e, sh, pos, orient,
+                       force, y6, vel, rot);
     return hipGetLastError();
 }
 

@@ -61,12 +61,12 @@ def test_parse_extension_lines(tmp_path):
     assert p.precision == rf.PREC_MIXED and p.warm_start == rf.WARM_LEARNED and p.profile == 1
     f.write_text(src + "precond auto\nprecond_switch_iters 45\nlu_factor_fp32 1\nrefine_tol 1d-13\n"
                        "precond_neighbours 6\nmixed_inner_tol 1e-11\nlu_block 96\nlu_update_fp16 1\n"
-                       "lu_column_panel 1\n")
+                       "lu_column_panel 1\nshear_rate 0.25\n")
     p = rf.Parameters.parse_parameter_file(str(f))
     assert (p.precond, p.precond_switch_iters, p.lu_factor_fp32, p.precond_neighbours, p.lu_block) == \
         (rf.PRECOND_AUTO, 45, 1, 6, 96)
     assert (p.lu_update_fp16, p.lu_column_panel) == (1, 1)
-    assert p.refine_tol == 1e-13 and p.mixed_inner_tol == 1e-11
+    assert p.refine_tol == 1e-13 and p.mixed_inner_tol == 1e-11 and p.shear_rate == 0.25
     for bad in ("precond sometimes\n", "warm_start later\n", "precision fp16\n", "lu_reuse_iters 4\n",
                 "refine_floor 1e-8\n"):
         f.write_text(src + bad)

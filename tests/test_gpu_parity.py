@@ -118,6 +118,48 @@ def test_velocities_vs_oracle(analytic):
     assert np.abs(w - wo).max() <= tol * max(np.abs(wo).max(), 1e-300)
 
 
+@pytest.mark.parametrize("precision", [rf.PREC_FP64, rf.PREC_MIXED])
+@pytest.mark.parametrize("analytic", [1, 0])
+def test_shear_velocities_vs_oracle(analytic, precision):
+    """Background shear (shear_rate y, 0, 0) in the velocities
+    (fortran/coeffs_and_vels.f90:597-600): the stored velocity operator (FP64)
+    and the matrix-free operator (mixed) against the oracle."""
+    N, M, noqi, sh = 30, 5, 8, 0.8
+    pos, ori = _layout(N, 1e-2, 9)
+    force = np.tile([0.2, -0.1, -1.0], N)
+    p = _params(analytic, 0, gmres_tol=1e-13, gmres_max_iters=300, external_force=(0.2, -0.1, -1.0),
+                shear_rate=sh, precision=precision)
+    with rf.Simulation(rf.Configuration(p, pos, ori)) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        x = sim.coefficients()
+        sim.update_velocities()
+        v, w = sim.velocities()
+        P, T = sim.state()
+    vo, wo = oracle.compute_velocities(analytic, EPS, P, T, force, M, noqi, x, sh=sh)
+    v0, w0 = oracle.compute_velocities(analytic, EPS, P, T, force, M, noqi, x, sh=0.0)
+    assert np.abs(v0 - vo).max() > 0.1 * np.abs(vo).max()  # the shear term is not negligible here
+    tol = 2e-10 if analytic else 1e-12
+    assert np.abs(v - vo).max() <= tol * np.abs(vo).max()
+    assert np.abs(w - wo).max() <= tol * np.abs(wo).max()
+
+
+def test_shear_k_steps_vs_oracle():
+    """K steps in a background shear against the oracle, 1e-9 relative."""
+    N, K, sh = 48, 6, 0.5
+    pos, ori = _layout(N, 1e-2, 23)
+    p = _params(1, 0, gmres_tol=1e-12, gmres_max_iters=300, shear_rate=sh)
+    with rf.Simulation(rf.Configuration(p, pos, ori)) as sim:
+        for _ in range(K):
+            sim.step()
+        P, T = sim.state()
+    ref = oracle.simulate(1, 0, EPS, 0.1, K, pos, ori, M=5, noqi=8, restart=10, max_iters=300, tol=1e-12, sh=sh)
+    still = oracle.simulate(1, 0, EPS, 0.1, K, pos, ori, M=5, noqi=8, restart=10, max_iters=300, tol=1e-12)
+    assert np.linalg.norm(still["pos"][-1] - ref["pos"][-1]) > 1e-3 * np.linalg.norm(ref["pos"][-1])
+    assert np.linalg.norm(P - ref["pos"][-1]) / np.linalg.norm(ref["pos"][-1]) <= 1e-9
+    assert np.linalg.norm(T - ref["orient"][-1]) / np.linalg.norm(ref["orient"][-1]) <= 1e-9
+
+
 def test_mfma_16x16x4_lane_layout():
     """v_mfma_{f64,f32}_16x16x4: lane l holds A[l%16][l/16] and B[l/16][l%16];
     accumulator r of lane l is D[l/16 + 4r][l%16] (f64) and D[4(l/16) + r][l%16]

@@ -43,7 +43,7 @@ def _handle(p, pos, ori, mode, x):
 
 @pytest.mark.parametrize("analytic", [1, 0])
 def test_noalloc_operator_matches_rf_operator(analytic):
-    p = rf.default_params(use_analytical_integration=analytic, external_force=(0.1, -0.2, -1.0))
+    p = rf.default_params(use_analytical_integration=analytic, external_force=(0.1, -0.2, -1.0), shear_rate=0.3)
     for N, seed in ((60, 71), (25, 72)):  # the second call smaller than the first
         pos, ori = layout(N, 2e-2, seed)
         rng = np.random.default_rng(seed)

@@ -10,27 +10,29 @@ from the previous step, LU fallback after 300 iterations
 (fortran/coeffs_and_vels.f90:253-339).
 A "step" is one full timestep (assemble, solve, velocities, advance).
 
-Beside it, on the same initial state: the mixed-precision + learned path
-(FP64 operator, residuals and Arnoldi; neighbour preconditioner switching to
-GMRES right-preconditioned by FP32 LU factors with fp16-operand trailing
-updates on hard systems; learned initial guess trained on the GPU on solved
-pairs), its
-relative position error against the plain run after the same steps, the FP64
-direct solver, and the same comparison at N=4000, volume fraction 5e-2, where
-the plain method is timed both as GMRES with LU fallback and as the FP64
-direct solver (the reference's two solver choices).
+Beside it, on the same initial state, the mixed-precision + learned path:
+the matrix assembled and stored in FP32 (FP64 near-field integrals), FP32 LU
+factors with fp16-operand trailing updates on the matrix cores as the
+preconditioner of hard systems, flexible GMRES on the matrix-free FP64
+operator (FP64 residuals and Arnoldi) around the FP32 solves, and the learned
+initial guess (network trained on the GPU on solved pairs).  Also timed: the
+same without the network, an FP64-matrix variant with FP32 LU factors, the
+FP64 direct solver, and the same comparison at N=4000, volume fraction 5e-2.
 
-Position differences are reported after one step and after all steps, both
-between the two methods and for each method against an FP64 run whose
-solves all reach a 1e-13 backward error: near fiber contacts the dynamics
-amplify any difference in a solve, so the plain method's own distance to
-that reference is what agreement after many steps can mean.
+Accuracy: the per-step ("shadow") position error.  The mixed + learned run's
+state before every timed step is restored into a handle of the plain FP64
+method (and of a 1e-13-tolerance FP64 method) and stepped once; the relative
+position difference after that step is the solver's own contribution,
+free of the chaotic amplification that near fiber contacts turns any 1e-10
+difference into O(1) after many steps.  `meets_accuracy` is its maximum over
+the timed steps <= 1e-6.  Position differences after all steps are reported
+beside it.
 
 Prints one JSON line.  `roofline` describes the dominant kernel of the
 headline workload (the system-matrix GEMV of the GMRES iterations), timed with
 HIP events on the library's stream inside the timed region.  `cpu_baseline`
 times the FP64 CPU oracle (a C restatement of the reference's Fortran) on the
-host cores on one step of the headline workload.
+host cores on warm-started steps of the headline workload.
 """
 import argparse
 import json
@@ -50,6 +52,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP16_MFMA_PEAK_TFLOPS = 2500.0  # dense
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemv_f64_n15000_traffic.json")
+ACCURACY = 1e-6
 
 C2 = dict(N=1000, phi=1e-2, seed=0x5EED0001)
 C4 = dict(N=4000, phi=5e-2, seed=0x5EED0003)
@@ -60,16 +63,19 @@ C4 = dict(N=4000, phi=5e-2, seed=0x5EED0003)
 # N=1000: cap 200 12.75, 300 12.86, 500 10.84, 1000 8.66 steps/s.  At N=4000
 # every step needs more and falls back (the direct solver is timed beside it)
 PLAIN = dict(gmres_restart=10, gmres_max_iters=300, gmres_tol=1e-10)
-# mixed precision + learned: FP64 operator, residuals and Arnoldi; FP32 LU
-# factors (trailing updates on fp16-rounded operands, FP32 accumulation, on
-# the matrix cores) as the preconditioner of hard systems (GMRES-IR) after
-# two neighbour-preconditioned restart cycles; learned initial guess
-IMPROVED = dict(gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-10, precond=3, lu_factor_fp32=1,
-                lu_update_fp16=1, warm_start=3)
-# the same without the network (previous-step guess)
-IMPROVED_NO_MLP = dict(IMPROVED, warm_start=0)
+PLAIN_DIRECT = dict(use_direct_solver=1)
+# mixed precision + learned: FP32 matrix, FP32 LU factors (fp16-operand
+# trailing updates) preconditioning the inner FP32 solves of hard systems,
+# flexible GMRES on the matrix-free FP64 operator, learned initial guess
+MIXED = dict(precision=1, gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-10, precond=3, lu_update_fp16=1,
+             warm_start=3)
+MIXED_NO_MLP = dict(MIXED, warm_start=0)
+# FP64 matrix with FP32 LU factors as GMRES-IR preconditioner, learned guess
+FP64_FP32LU = dict(gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-10, precond=3, lu_factor_fp32=1,
+                   lu_update_fp16=1, warm_start=3)
 # FP64 reference for the error figures: every solve to a 1e-13 backward error
-TIGHT = dict(IMPROVED_NO_MLP, gmres_tol=1e-13, refine_tol=1e-13)
+TIGHT = dict(gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-13, refine_tol=1e-13, precond=3,
+             lu_factor_fp32=1, lu_update_fp16=1)
 
 
 def analytic_assembly_flops(pos, ori, M=5, noqi=8):
@@ -115,11 +121,14 @@ def _stats_delta(s0, s1, steps):
         "velocity_gemv_avg_ms": d("velocity_gemv_ms_total") / max(1, d("velocity_gemv_launches")),
         "lu_factorizations": d("lu_factorizations"),
         "lu_avg_ms": d("lu_ms_total") / max(1, d("lu_factorizations")),
+        "operator_launches_per_step": d("operator_launches") / steps,
+        "operator_avg_ms": d("operator_ms_total") / max(1, d("operator_launches")),
     }
 
 
 def timed_run(rf, pos, ori, steps, warmup=0, weights=None, **kw):
-    """K timed steps after W untimed ones; returns (record, final positions)."""
+    """K timed steps after W untimed ones; returns (record, positions after
+    all steps)."""
     sim = rf.Simulation(rf.Configuration(rf.default_params(profile=1, **kw), pos, ori))
     if weights is not None:
         sim.mlp_set_weights(weights)
@@ -128,11 +137,8 @@ def timed_run(rf, pos, ori, steps, warmup=0, weights=None, **kw):
     sim.synchronize()
     s0 = sim.stats()
     t0 = time.perf_counter()
-    first = None
-    for k in range(steps):
+    for _ in range(steps):
         sim.step()
-        if k == 0:  # positions after the first timed step (one 3N copy)
-            first, _ = sim.state()
     sim.synchronize()
     wall = time.perf_counter() - t0
     s1 = sim.stats()
@@ -141,12 +147,50 @@ def timed_run(rf, pos, ori, steps, warmup=0, weights=None, **kw):
     out.update(_stats_delta(s0, s1, steps))
     P, _ = sim.state()
     sim.close()
-    return out, (P, first)
+    return out, P
+
+
+def shadow(rf, pos, ori, steps, warmup, weights, kw, refs):
+    """Per-step position error of the method `kw` against each reference
+    method restarted from the same state: the method's run is replayed
+    (untimed) and its state before every step recorded; each reference
+    handle then restores every recorded state and steps once.  Returns
+    {ref: [relative position difference after each step]} and the same
+    between the references (the references' own disagreement)."""
+    sim = rf.Simulation(rf.Configuration(rf.default_params(**kw), pos, ori))
+    if weights is not None:
+        sim.mlp_set_weights(weights)
+    for _ in range(warmup):
+        sim.step()
+    states, after = [], []
+    for _ in range(steps):
+        states.append(sim.restart_state())
+        sim.step()
+        after.append(sim.state()[0])
+    sim.close()
+    stepped = {}
+    for name, rkw in refs.items():
+        h = rf.Simulation(rf.Configuration(rf.default_params(**rkw), pos, ori))
+        out = []
+        for S in states:
+            h.restore_state(S)
+            h.step()
+            out.append(h.state()[0])
+        h.close()
+        stepped[name] = out
+    names = list(refs)
+    res = {"method_vs_" + n: [rel(a, b) for a, b in zip(after, stepped[n])] for n in names}
+    for i, a in enumerate(names):
+        for b in names[i + 1:]:
+            res[a + "_vs_" + b] = [rel(x, y) for x, y in zip(stepped[a], stepped[b])]
+    return res
 
 
 def train_guess(rf, pos, ori, steps, epochs=30, **kw):
     """Solved (configuration, coefficients) pairs along `steps` steps from
-    (pos, ori), then GPU training of the network; returns (weights, info)."""
+    (pos, ori), then GPU training of the network on all but the last state;
+    the last state's pairs are held out.  Returns (weights, info, positions,
+    orientations after the steps)."""
     sim = rf.Simulation(rf.Configuration(rf.default_params(**kw), pos, ori))
     Xs, Ys = [], []
     t0 = time.perf_counter()
@@ -159,29 +203,18 @@ def train_guess(rf, pos, ori, steps, epochs=30, **kw):
         sim.update_velocities()
         sim.update_fibers()
     t1 = time.perf_counter()
-    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    X, Y = np.concatenate(Xs[:-1]), np.concatenate(Ys[:-1])
     hist = sim.mlp_train(X, Y, epochs=epochs, batch=256, lr=1e-3, seed=1)
     sim.synchronize()
     t2 = time.perf_counter()
+    held_out, _ = sim.mlp_forward_backward(Xs[-1], Ys[-1])
     w = sim.mlp_weights()
     P, T = sim.state()
     sim.close()
-    return w, {"samples": int(X.shape[0]), "epochs": epochs, "sample_s": t1 - t0, "train_s": t2 - t1,
-               "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1])}, P, T
-
-
-def errors(Xi, Xp, Xt):
-    """Relative position differences after the first and after all timed
-    steps: mixed + learned vs plain, and each against the tight-tolerance
-    FP64 run."""
-    return {
-        "position_rel_err": rel(Xi[0], Xp[0]),
-        "position_rel_err_one_step": rel(Xi[1], Xp[1]),
-        "plain_vs_tight_rel_err": rel(Xp[0], Xt[0]),
-        "mixed_learned_vs_tight_rel_err": rel(Xi[0], Xt[0]),
-        "plain_vs_tight_rel_err_one_step": rel(Xp[1], Xt[1]),
-        "mixed_learned_vs_tight_rel_err_one_step": rel(Xi[1], Xt[1]),
-    }
+    return w, {"samples": int(X.shape[0]), "held_out_samples": int(Xs[-1].shape[0]), "epochs": epochs,
+               "sample_s": t1 - t0, "train_s": t2 - t1, "loss_first_epoch": float(hist[0]),
+               "loss_last_epoch": float(hist[-1]), "held_out_loss": float(held_out),
+               "loss_unit": "mean squared error of standardised coefficients (1 = predicting the mean)"}, P, T
 
 
 def log(msg):
@@ -193,51 +226,92 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def compare(rf, cfg, steps, warmup, train_states, label):
-    """Plain FP64 and mixed + learned on the same state: steps/s, ratio and
-    position error after the same number of steps, next to the error of a run
-    at a 100x tighter tolerance (the trajectories are chaotic near fiber
-    contacts: any 1e-10 difference in a solve grows step by step)."""
+def accuracy_record(sh, final_pos):
+    """Shadow maxima and the positions after all steps."""
+    mx = {k: max(v) for k, v in sh.items()}
+    out = {
+        "shadow_max_rel_err": mx["method_vs_plain"],
+        "shadow_max_rel_err_vs_tight": mx["method_vs_tight"],
+        "shadow_plain_vs_tight_max_rel_err": mx["plain_vs_tight"],
+        "shadow_per_step": {k: v for k, v in sh.items()},
+        "meets_accuracy": bool(mx["method_vs_plain"] <= ACCURACY),
+        "accuracy_bound": ACCURACY,
+    }
+    Pm, Pp = final_pos
+    out["position_rel_err_after_all_steps"] = rel(Pm, Pp)
+    return out
+
+
+def compare(rf, cfg, steps, warmup, train_states, label, plain_ref):
+    """Plain FP64 and mixed + learned on the same state (after the training
+    states): steps/s, per-step shadow errors and positions after all steps."""
     pos, ori = layout(cfg["N"], cfg["phi"], cfg["seed"])
-    # the network is trained on solved pairs of the first states of the
-    # trajectory; the timed runs start where the training states end
-    w, tinfo, P0, T0 = train_guess(rf, pos, ori, train_states, **IMPROVED_NO_MLP)
     tag = label.split(",")[0]
-    log("%s: network trained on %d pairs" % (tag, tinfo["samples"]))
-    plain, Xp = timed_run(rf, P0, T0, steps, warmup, **PLAIN)
+    w, tinfo, P0, T0 = train_guess(rf, pos, ori, train_states, **MIXED_NO_MLP)
+    log("%s: network trained on %d pairs, held-out loss %.3f" % (tag, tinfo["samples"], tinfo["held_out_loss"]))
+    out = {"workload": label, "training": tinfo}
+    plain, Pp = timed_run(rf, P0, T0, steps, warmup, **PLAIN)
     log("%s: plain FP64 %.3f steps/s" % (tag, plain["steps_per_sec"]))
-    direct, _ = timed_run(rf, P0, T0, steps, warmup, use_direct_solver=1)
+    direct, Pd = timed_run(rf, P0, T0, steps, warmup, **PLAIN_DIRECT)
     log("%s: plain FP64 direct solver %.3f steps/s" % (tag, direct["steps_per_sec"]))
-    impr, Xi = timed_run(rf, P0, T0, steps, warmup, weights=w, **IMPROVED)
-    log("%s: mixed + learned %.3f steps/s" % (tag, impr["steps_per_sec"]))
-    _, Xt = timed_run(rf, P0, T0, steps, warmup, **TIGHT)
-    return {
-        "workload": label,
+    mixed, Pm = timed_run(rf, P0, T0, steps, warmup, weights=w, **MIXED)
+    log("%s: mixed + learned %.3f steps/s" % (tag, mixed["steps_per_sec"]))
+    nomlp, _ = timed_run(rf, P0, T0, steps, warmup, **MIXED_NO_MLP)
+    log("%s: mixed without the network %.3f steps/s" % (tag, nomlp["steps_per_sec"]))
+    f64, _ = timed_run(rf, P0, T0, steps, warmup, weights=w, **FP64_FP32LU)
+    log("%s: FP64 matrix + FP32 LU + learned %.3f steps/s" % (tag, f64["steps_per_sec"]))
+    ref = PLAIN_DIRECT if plain_ref == "direct" else PLAIN
+    sh = shadow(rf, P0, T0, steps, warmup, w, MIXED, {"plain": ref, "tight": TIGHT})
+    log("%s: shadow error max %.2e" % (tag, max(sh["method_vs_plain"])))
+    best_plain = max(plain["steps_per_sec"], direct["steps_per_sec"])
+    out.update({
         "plain_fp64": plain,
         "plain_fp64_direct": direct,
-        "mixed_learned": impr,
-        "speedup": impr["steps_per_sec"] / plain["steps_per_sec"],
-        "speedup_vs_plain_direct": impr["steps_per_sec"] / direct["steps_per_sec"],
-        "training": tinfo,
-        **errors(Xi, Xp, Xt),
-    }
+        "mixed_learned": mixed,
+        "mixed_without_network": {k: nomlp[k] for k in ("steps_per_sec", "gmres_iters_per_step",
+                                                        "operator_launches_per_step", "lu_factorizations")},
+        "fp64_matrix_fp32_lu_learned": f64,
+        "speedup": mixed["steps_per_sec"] / plain["steps_per_sec"],
+        "speedup_vs_best_plain": mixed["steps_per_sec"] / best_plain,
+        "shadow_reference": "plain FP64 %s" % ("direct solver" if plain_ref == "direct" else "GMRES(10), tol 1e-10"),
+    })
+    out.update(accuracy_record(sh, (Pm, Pd if plain_ref == "direct" else Pp)))
+    return out
 
 
-def cpu_baseline(N, phi, seed, threads):
+def cpu_baseline(N, phi, seed, threads, warmup=2, steps=5):
+    """The FP64 C oracle on warm-started steps of the headline workload:
+    time(warmup + steps) - time(warmup) from the same initial state."""
     import oracle
 
     os.environ["OMP_NUM_THREADS"] = str(threads)
     pos, ori = layout(N, phi, seed)
+    kw = dict(M=5, noqi=8, restart=10, max_iters=300, tol=1e-10)
     t0 = time.perf_counter()
-    oracle.simulate(True, False, 0.01, 0.1, 1, pos, ori, M=5, noqi=8, restart=10, max_iters=2000, tol=1e-10)
-    dt = time.perf_counter() - t0
+    oracle.simulate(True, False, 0.01, 0.1, warmup, pos, ori, **kw)
+    t1 = time.perf_counter()
+    oracle.simulate(True, False, 0.01, 0.1, warmup + steps, pos, ori, **kw)
+    t2 = time.perf_counter()
+    dt = (t2 - t1) - (t1 - t0)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
-        "value": 1.0 / dt,
+        "value": steps / dt,
         "unit": "steps/s",
         "cores": int(oracle.num_threads()),
         "kind": "port",
-        "sample": "1 timestep of the N=1000 headline workload (analytic assembly, GMRES(10) to 1e-10 from x0=0, "
-                  "velocities, advance) with the FP64 C oracle, OpenMP over target fibers; %.1f s" % dt,
+        "cpu_model": model,
+        "sample": "steps %d..%d of the N=1000 headline workload (analytic assembly, GMRES(10) to 1e-10 warm-started "
+                  "from the previous step with LU fallback after 300 iterations, velocities, advance) with the FP64 C "
+                  "oracle, OpenMP over target fibers: time of %d steps minus time of the first %d, %.1f s for the %d "
+                  "steps" % (warmup + 1, warmup + steps, warmup + steps, warmup, dt, steps),
     }
 
 
@@ -258,7 +332,7 @@ def main():
     import rigidfibers_kth_amd as rf
 
     pos, ori = layout(C2["N"], C2["phi"], C2["seed"])
-    g, Xp = timed_run(rf, pos, ori, args.steps, args.warmup, **PLAIN)
+    g, Pp = timed_run(rf, pos, ori, args.steps, args.warmup, **PLAIN)
     log("N=1000 plain FP64: %.2f steps/s" % g["steps_per_sec"])
     n, eb = g["n"], g["elem_bytes"]
     gemv_bytes = float(n) * n * eb + 2.0 * n * 8  # matrix once + x and y
@@ -276,17 +350,21 @@ def main():
 
     # mixed + learned at N=1000 on the same initial state and step count; the
     # network is trained on solved pairs of another seeded layout
-    w, tinfo, _, _ = train_guess(rf, *layout(C2["N"], C2["phi"], C2["seed"] + 100), 6, **IMPROVED_NO_MLP)
-    impr, Xi = timed_run(rf, pos, ori, args.steps, args.warmup, weights=w, **IMPROVED)
-    log("N=1000 mixed + learned: %.2f steps/s" % impr["steps_per_sec"])
-    nomlp, _ = timed_run(rf, pos, ori, args.steps, args.warmup, **IMPROVED_NO_MLP)
-    _, Xt = timed_run(rf, pos, ori, args.steps, args.warmup, **TIGHT)
-    log("N=1000 tight-tolerance reference run done")
+    w, tinfo, _, _ = train_guess(rf, *layout(C2["N"], C2["phi"], C2["seed"] + 100), 7, **MIXED_NO_MLP)
+    mixed, Pm = timed_run(rf, pos, ori, args.steps, args.warmup, weights=w, **MIXED)
+    log("N=1000 mixed + learned: %.2f steps/s" % mixed["steps_per_sec"])
+    nomlp, _ = timed_run(rf, pos, ori, args.steps, args.warmup, **MIXED_NO_MLP)
+    f64, _ = timed_run(rf, pos, ori, args.steps, args.warmup, weights=w, **FP64_FP32LU)
+    sh = shadow(rf, pos, ori, args.steps, args.warmup, w, MIXED, {"plain": PLAIN, "tight": TIGHT})
+    log("N=1000 shadow error max %.2e" % max(sh["method_vs_plain"]))
+    n1000 = dict(mixed, speedup=mixed["steps_per_sec"] / g["steps_per_sec"], training=tinfo)
+    n1000.update(accuracy_record(sh, (Pm, Pp)))
     extra = {
-        "n1000_mixed_learned": dict(impr, speedup=impr["steps_per_sec"] / g["steps_per_sec"], training=tinfo,
-                                    **errors(Xi, Xp, Xt)),
-        "n1000_without_network": {"steps_per_sec": nomlp["steps_per_sec"],
-                                  "gmres_iters_per_step": nomlp["gmres_iters_per_step"]},
+        "n1000_mixed_learned": n1000,
+        "n1000_mixed_without_network": {k: nomlp[k] for k in ("steps_per_sec", "gmres_iters_per_step",
+                                                              "operator_launches_per_step")},
+        "n1000_fp64_matrix_fp32_lu_learned": {k: f64[k] for k in ("steps_per_sec", "gmres_iters_per_step",
+                                                                  "lu_factorizations")},
     }
     if not args.no_lu:
         lu, _ = timed_run(rf, pos, ori, 2, 1, use_direct_solver=1)
@@ -298,8 +376,8 @@ def main():
             "frac_fp64_peak": (2.0 / 3.0) * n ** 3 / (lu["lu_avg_ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         }
     if not args.no_n4000:
-        c4 = compare(rf, C4, args.n4000_steps, 0, 3, "N=4000, volume fraction 5e-2, M=5, dt=0.1, tol 1e-10; "
-                     "timed steps start after 3 solved training states")
+        c4 = compare(rf, C4, args.n4000_steps, 0, 4, "N=4000, volume fraction 5e-2, M=5, dt=0.1, tol 1e-10; "
+                     "timed steps start after 4 solved training states", "direct")
         mi = c4["mixed_learned"]
         if mi["lu_factorizations"]:
             n4 = mi["n"]
@@ -348,8 +426,9 @@ def main():
     }
     line.update(extra)
     if not args.no_cpu:
-        log("CPU oracle step")
-        line["cpu_baseline"] = cpu_baseline(C2["N"], C2["phi"], C2["seed"], min(16, os.cpu_count() or 1))
+        log("CPU oracle steps")
+        threads = min(16, len(os.sched_getaffinity(0)))
+        line["cpu_baseline"] = cpu_baseline(C2["N"], C2["phi"], C2["seed"], threads)
     print(json.dumps(line), flush=True)
 
 

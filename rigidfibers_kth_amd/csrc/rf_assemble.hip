@@ -72,6 +72,7 @@ __global__ void __launch_bounds__(256, 2) assemble_kernel(const KArgs a)
     const int gstride = LQ + 1;
     double* sG = smem;                                    // [T*CH*6][LQ+1]
     double* sTh = smem + (size_t)T * CH * 6 * gstride;    // [T][NL k][NL l][6]
+    double* sW = sTh + (size_t)T * NL * NL * 6;           // [LQ][NL]: w_p P_k(s_p)
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int i0 = blockIdx.x * T;
@@ -90,6 +91,12 @@ __global__ void __launch_bounds__(256, 2) assemble_kernel(const KArgs a)
         X1 = a.pos[3 * i1 + 1] + s * a.orient[3 * i1 + 1];
         X2 = a.pos[3 * i1 + 2] + s * a.orient[3 * i1 + 2];
     }
+    // the Legendre weights of the Theta contraction, once per workgroup
+    for (int q = tid; q < LQ * NL; q += nthr) {
+        const int p = q / NL, k = q - p * NL;
+        sW[q] = a.wp[k * LQ + p];
+    }
+    // (the first barrier of the source loop orders these stores before the reads)
     const int nA = T * 3 * M;          // A-row items
     const int nItems = nA + (a.write_w ? T * 6 : 0);  // + W-row items
     double bacc[kMaxItems];
@@ -156,8 +163,9 @@ __global__ void __launch_bounds__(256, 2) assemble_kernel(const KArgs a)
                 for (int k = 0; k < NL; ++k) acc[k] = 0.0;
                 for (int p = 0; p < LQ; ++p) {
                     const double gp = gr[p];
+                    const double* wk = sW + p * NL;
 #pragma unroll
-                    for (int k = 0; k < NL; ++k) acc[k] += a.wp[k * LQ + p] * gp;
+                    for (int k = 0; k < NL; ++k) acc[k] += wk[k] * gp;
                 }
 #pragma unroll
                 for (int k = 0; k < NL; ++k) sTh[((t * NL + k) * NL + l) * 6 + c] = acc[k];
@@ -181,7 +189,7 @@ __global__ void __launch_bounds__(256, 2) assemble_kernel(const KArgs a)
 #pragma unroll
                         for (int bc = 0; bc < 3; ++bc) {
                             const int64_t col = (int64_t)j * 3 * M + (l - 1) * 3 + bc;
-                            A[col * a.lda + row] = (AT)((k == l && ar == bc) ? 1.0 : 0.0);
+                            __builtin_nontemporal_store((AT)((k == l && ar == bc) ? 1.0 : 0.0), &A[col * a.lda + row]);
                         }
                     continue;
                 }
@@ -195,7 +203,7 @@ __global__ void __launch_bounds__(256, 2) assemble_kernel(const KArgs a)
                         const double Qb = tl[sym6(bc, 0)] * ti[0] + tl[sym6(bc, 1)] * ti[1] + tl[sym6(bc, 2)] * ti[2];
                         const double v = (k == 1) ? a.D1 * ti[ar] * Qb : gk * (tl[sym6(ar, bc)] - Ek * ti[ar] * Qb);
                         const int64_t col = (int64_t)j * 3 * M + (l - 1) * 3 + bc;
-                        A[col * a.lda + row] = (AT)v;
+                        __builtin_nontemporal_store((AT)v, &A[col * a.lda + row]);
                     }
                 }
                 // rhs: contr = Theta_{k0} f0_j
@@ -226,7 +234,7 @@ __global__ void __launch_bounds__(256, 2) assemble_kernel(const KArgs a)
 #pragma unroll
                     for (int c2 = 0; c2 < 3; ++c2) {
                         const int64_t col = (int64_t)j * 3 * NL + l * 3 + c2;
-                        W[col * a.ldw + row] = (AT)((i == j) ? 0.0 : th[l * 6 + sym6(c, c2)]);
+                        __builtin_nontemporal_store((AT)((i == j) ? 0.0 : th[l * 6 + sym6(c, c2)]), &W[col * a.ldw + row]);
                     }
             }
         }
@@ -266,7 +274,7 @@ size_t assemble_lds_bytes(int T, int LQ, int M, bool analytic)
 {
     const size_t NL = M + 1;
     const size_t CH = analytic ? (M + 2) / 2 : M + 1;
-    return sizeof(double) * ((size_t)T * CH * 6 * (LQ + 1) + (size_t)T * NL * NL * 6);
+    return sizeof(double) * ((size_t)T * CH * 6 * (LQ + 1) + (size_t)T * NL * NL * 6 + (size_t)LQ * NL);
 }
 
 int assemble_targets_per_block(int LQ, int M, bool analytic)

@@ -112,12 +112,14 @@ def test_zero_column_sets_info():
     assert info == 38 and fb >= 1  # the zero pivot sends its panel to the column search
 
 
-@pytest.mark.parametrize("n,nb", [(515, 0), (300, 100), (200, 13)])
+@pytest.mark.parametrize("n,nb", [(515, 0), (300, 100), (200, 13), (5500, 0)])
 def test_fp16_update_factors(n, nb):
     """FP32 factors with fp16-rounded trailing-update operands: the pivots of
     partial pivoting and a backward error of the order of fp16 rounding;
     scaled copies of the matrix (2^-30, 2^30) give the same relative error
-    (per-column scaling of U12 keeps fp16 away from underflow and overflow)."""
+    (per-column scaling of U12 keeps fp16 away from underflow and overflow).
+    n = 5500 reaches the 256 x 256 tile update (trailing matrix >= 4096 after
+    the first 1024-column super-panel)."""
     A = _dominant_with_block_swaps(n, nb if nb else 64, 17 + n)
     A = A.astype(np.float32).astype(np.float64)
     _, piv_ref = sl.lu_factor(A)

@@ -186,35 +186,73 @@ def shadow(rf, pos, ori, steps, warmup, weights, kw, refs):
     return res
 
 
-def train_guess(rf, pos, ori, steps, epochs=30, **kw):
-    """Solved (configuration, coefficients) pairs along `steps` steps from
-    (pos, ori), then GPU training of the network on all but the last state;
-    the last state's pairs are held out.  Returns (weights, info, positions,
-    orientations after the steps)."""
-    sim = rf.Simulation(rf.Configuration(rf.default_params(**kw), pos, ori))
+# the plain FP64 solution of the training states (FP64 matrix; FP32 LU factors
+# only precondition GMRES, which still solves the FP64 system to 1e-10)
+TRAIN_SOLVE = dict(gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-10, precond=3, lu_factor_fp32=1,
+                   lu_update_fp16=1)
+
+
+def train_network(rf, N, phi, seeds, states, held_pos, held_ori, epochs=40):
+    """Solved (configuration, coefficients) pairs from `states` steps of each
+    seeded layout (N fibers, volume fraction phi), GPU training of the
+    network, and its loss on the pairs of a held-out configuration (the
+    benchmark's own initial state, never trained on).  Returns (weights,
+    info)."""
     Xs, Ys = [], []
     t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.assemble_system()
-        sim.solve_system()
-        X, Y = sim.mlp_samples()
-        Xs.append(X)
-        Ys.append(Y)
-        sim.update_velocities()
-        sim.update_fibers()
+    for sd in seeds:
+        pos, ori = layout(N, phi, sd)
+        sim = rf.Simulation(rf.Configuration(rf.default_params(**TRAIN_SOLVE), pos, ori))
+        for _ in range(states):
+            sim.assemble_system()
+            sim.solve_system()
+            X, Y = sim.mlp_samples()
+            Xs.append(X)
+            Ys.append(Y)
+            sim.update_velocities()
+            sim.update_fibers()
+        sim.close()
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
     t1 = time.perf_counter()
-    X, Y = np.concatenate(Xs[:-1]), np.concatenate(Ys[:-1])
+    sim = rf.Simulation(rf.Configuration(rf.default_params(**TRAIN_SOLVE), held_pos, held_ori))
+    sim.assemble_system()
+    sim.solve_system()
+    Xh, Yh = sim.mlp_samples()
+    t2 = time.perf_counter()
     hist = sim.mlp_train(X, Y, epochs=epochs, batch=256, lr=1e-3, seed=1)
     sim.synchronize()
-    t2 = time.perf_counter()
-    held_out, _ = sim.mlp_forward_backward(Xs[-1], Ys[-1])
+    t3 = time.perf_counter()
+    held_out, _ = sim.mlp_forward_backward(Xh, Yh)
     w = sim.mlp_weights()
-    P, T = sim.state()
     sim.close()
-    return w, {"samples": int(X.shape[0]), "held_out_samples": int(Xs[-1].shape[0]), "epochs": epochs,
-               "sample_s": t1 - t0, "train_s": t2 - t1, "loss_first_epoch": float(hist[0]),
-               "loss_last_epoch": float(hist[-1]), "held_out_loss": float(held_out),
-               "loss_unit": "mean squared error of standardised coefficients (1 = predicting the mean)"}, P, T
+    return w, {"pairs": int(X.shape[0]), "layouts": len(seeds), "states_per_layout": states,
+               "layout_fibers": N, "volume_fraction": phi, "held_out_pairs": int(Xh.shape[0]),
+               "epochs": epochs, "sample_s": t1 - t0, "train_s": t3 - t2,
+               "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1]),
+               "held_out_loss": float(held_out),
+               "loss_unit": "mean squared error of standardised coefficients (1 = predicting the mean)"}
+
+
+def guess_quality(rf, pos, ori, steps, weights, kw):
+    """Per step of `kw` (mixed path) with the learned guess and with the
+    previous-step guess: the FP64 backward error of the initial guess and the
+    inner iterations (untimed)."""
+    out = {}
+    for name, ws in (("learned", 3), ("previous_step", 0)):
+        sim = rf.Simulation(rf.Configuration(rf.default_params(**dict(kw, warm_start=ws)), pos, ori))
+        if ws == 3:
+            sim.mlp_set_weights(weights)
+        be, it = [], []
+        for _ in range(steps):
+            sim.step()
+            st = sim.stats()
+            be.append(float(st.refine_history[0]))
+            it.append(int(st.last_gmres_iters))
+        sim.close()
+        out[name] = {"initial_backward_error": be, "inner_iterations": it}
+    out["mean_ratio_learned_to_previous_step"] = float(np.mean(out["learned"]["initial_backward_error"]) /
+                                                       np.mean(out["previous_step"]["initial_backward_error"]))
+    return out
 
 
 def log(msg):
@@ -242,13 +280,13 @@ def accuracy_record(sh, final_pos):
     return out
 
 
-def compare(rf, cfg, steps, warmup, train_states, label, plain_ref):
-    """Plain FP64 and mixed + learned on the same state (after the training
-    states): steps/s, per-step shadow errors and positions after all steps."""
-    pos, ori = layout(cfg["N"], cfg["phi"], cfg["seed"])
+def compare(rf, cfg, steps, warmup, train, label, plain_ref):
+    """Plain FP64 and mixed + learned on the same initial state: steps/s,
+    per-step shadow errors, guess quality and positions after all steps."""
+    P0, T0 = layout(cfg["N"], cfg["phi"], cfg["seed"])
     tag = label.split(",")[0]
-    w, tinfo, P0, T0 = train_guess(rf, pos, ori, train_states, **MIXED_NO_MLP)
-    log("%s: network trained on %d pairs, held-out loss %.3f" % (tag, tinfo["samples"], tinfo["held_out_loss"]))
+    w, tinfo = train_network(rf, train["N"], cfg["phi"], train["seeds"], 2, P0, T0)
+    log("%s: network trained on %d pairs, held-out loss %.3f" % (tag, tinfo["pairs"], tinfo["held_out_loss"]))
     out = {"workload": label, "training": tinfo}
     plain, Pp = timed_run(rf, P0, T0, steps, warmup, **PLAIN)
     log("%s: plain FP64 %.3f steps/s" % (tag, plain["steps_per_sec"]))
@@ -263,6 +301,7 @@ def compare(rf, cfg, steps, warmup, train_states, label, plain_ref):
     ref = PLAIN_DIRECT if plain_ref == "direct" else PLAIN
     sh = shadow(rf, P0, T0, steps, warmup, w, MIXED, {"plain": ref, "tight": TIGHT})
     log("%s: shadow error max %.2e" % (tag, max(sh["method_vs_plain"])))
+    out["guess_quality"] = guess_quality(rf, P0, T0, steps, w, MIXED)
     best_plain = max(plain["steps_per_sec"], direct["steps_per_sec"])
     out.update({
         "plain_fp64": plain,
@@ -349,8 +388,9 @@ def main():
     asm_tflops = flops / (g["assemble_avg_ms"] * 1e-3) / 1e12
 
     # mixed + learned at N=1000 on the same initial state and step count; the
-    # network is trained on solved pairs of another seeded layout
-    w, tinfo, _, _ = train_guess(rf, *layout(C2["N"], C2["phi"], C2["seed"] + 100), 7, **MIXED_NO_MLP)
+    # network is trained on solved pairs of ten other seeded layouts
+    w, tinfo = train_network(rf, C2["N"], C2["phi"], [C2["seed"] + 100 + k for k in range(10)], 2, pos, ori)
+    log("N=1000: network trained on %d pairs, held-out loss %.3f" % (tinfo["pairs"], tinfo["held_out_loss"]))
     mixed, Pm = timed_run(rf, pos, ori, args.steps, args.warmup, weights=w, **MIXED)
     log("N=1000 mixed + learned: %.2f steps/s" % mixed["steps_per_sec"])
     nomlp, _ = timed_run(rf, pos, ori, args.steps, args.warmup, **MIXED_NO_MLP)
@@ -359,6 +399,7 @@ def main():
     log("N=1000 shadow error max %.2e" % max(sh["method_vs_plain"]))
     n1000 = dict(mixed, speedup=mixed["steps_per_sec"] / g["steps_per_sec"], training=tinfo)
     n1000.update(accuracy_record(sh, (Pm, Pp)))
+    n1000["guess_quality"] = guess_quality(rf, pos, ori, 10, w, MIXED)
     extra = {
         "n1000_mixed_learned": n1000,
         "n1000_mixed_without_network": {k: nomlp[k] for k in ("steps_per_sec", "gmres_iters_per_step",
@@ -376,8 +417,10 @@ def main():
             "frac_fp64_peak": (2.0 / 3.0) * n ** 3 / (lu["lu_avg_ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         }
     if not args.no_n4000:
-        c4 = compare(rf, C4, args.n4000_steps, 0, 4, "N=4000, volume fraction 5e-2, M=5, dt=0.1, tol 1e-10; "
-                     "timed steps start after 4 solved training states", "direct")
+        c4 = compare(rf, C4, args.n4000_steps, 0, dict(N=2000, seeds=[C4["seed"] + 0x100 + k for k in range(5)]),
+                     "N=4000, volume fraction 5e-2, M=5, dt=0.1, tol 1e-10, from the seeded initial layout; network "
+                     "trained on 2 states of 5 other seeded layouts of 2000 fibers at the same volume fraction",
+                     "direct")
         mi = c4["mixed_learned"]
         if mi["lu_factorizations"]:
             n4 = mi["n"]

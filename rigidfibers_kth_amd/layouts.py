@@ -44,7 +44,13 @@ def layout(N, phi, seed, length=2.0, eps=0.01, min_gap=None):
             ph = rng.uniform(0.0, 2 * np.pi)
             s = np.sqrt(1 - z * z)
             t = np.array([s * np.cos(ph), s * np.sin(ph), z])
-            if i == 0 or gap <= 0 or segment_distances(p, t, pos[:i], ori[:i], 0.5 * length).min() >= gap:
+            if i == 0 or gap <= 0:
+                break
+            # only fibers whose centres are closer than length + gap can come
+            # within gap (the margin keeps every decision that of the full check)
+            d = pos[:i] - p
+            near = np.einsum("ij,ij->i", d, d) < (length + gap + 1e-9) ** 2
+            if not near.any() or segment_distances(p, t, pos[:i][near], ori[:i][near], 0.5 * length).min() >= gap:
                 break
         pos[i] = p
         ori[i] = t

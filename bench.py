@@ -318,6 +318,49 @@ def compare(rf, cfg, steps, warmup, train, label, plain_ref):
     return out
 
 
+def long_run(rf, steps):
+    """`--n4000-long K`: K steps at N=4000 from the seeded layout with the
+    plain FP64 direct solver, the mixed + learned path and the 1e-13 FP64
+    reference; steps/s of the first two and the relative position
+    differences at checkpoints (progress on stderr every 10 steps)."""
+    P0, T0 = layout(C4["N"], C4["phi"], C4["seed"])
+    w, tinfo = train_network(rf, 2000, C4["phi"], [C4["seed"] + 0x100 + k for k in range(5)], 2, P0, T0)
+    marks = sorted({k for k in (1, 2, 3, 5, 10, 20, 30, 50, 70, 100, steps) if k <= steps})
+    runs = {}
+    for name, kw, wts in (("mixed_learned", MIXED, w), ("plain_fp64_direct", PLAIN_DIRECT, None),
+                          ("tight_fp64", TIGHT, None)):
+        sim = rf.Simulation(rf.Configuration(rf.default_params(profile=1, **kw), P0, T0))
+        if wts is not None:
+            sim.mlp_set_weights(wts)
+        snaps, t0 = {}, time.perf_counter()
+        for k in range(1, steps + 1):
+            sim.step()
+            if k in marks:
+                snaps[k] = sim.state()[0]
+            if k % 10 == 0:
+                log("%s: %d steps, %.1f s" % (name, k, time.perf_counter() - t0))
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        st = sim.stats()
+        sim.close()
+        runs[name] = {"snaps": snaps, "steps_per_sec": steps / wall, "lu_factorizations": st.lu_factorizations,
+                      "gmres_iters": st.total_gmres_iters}
+    diffs = {}
+    for a, b in (("mixed_learned", "plain_fp64_direct"), ("plain_fp64_direct", "tight_fp64"),
+                 ("mixed_learned", "tight_fp64")):
+        diffs["%s_vs_%s" % (a, b)] = {str(k): rel(runs[a]["snaps"][k], runs[b]["snaps"][k]) for k in marks}
+    return {
+        "workload": "N=4000, volume fraction 5e-2, M=5, dt=0.1, %d steps from the seeded layout (seed 0x%X)"
+                    % (steps, C4["seed"]),
+        "steps": steps,
+        "training": tinfo,
+        "steps_per_sec": {k: v["steps_per_sec"] for k, v in runs.items()},
+        "speedup_vs_plain_direct": runs["mixed_learned"]["steps_per_sec"] / runs["plain_fp64_direct"]["steps_per_sec"],
+        "lu_factorizations": {k: v["lu_factorizations"] for k, v in runs.items()},
+        "position_rel_diff_at_step": diffs,
+    }
+
+
 def cpu_baseline(N, phi, seed, threads, warmup=2, steps=5):
     """The FP64 C oracle on warm-started steps of the headline workload:
     time(warmup + steps) - time(warmup) from the same initial state."""
@@ -363,6 +406,9 @@ def main():
     ap.add_argument("--no-lu", action="store_true", help="skip the direct-solver timing")
     ap.add_argument("--no-n4000", action="store_true", help="skip the N=4000 comparison")
     ap.add_argument("--n4000-steps", type=int, default=3)
+    ap.add_argument("--n4000-long", type=int, default=0, metavar="K",
+                    help="instead of the benchmark: K steps at N=4000 of the plain direct solver, mixed + learned "
+                         "and the 1e-13 reference with position differences at checkpoints (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the multi-GPU row sharding is a design only "
@@ -370,6 +416,9 @@ def main():
         sys.exit(2)
     import rigidfibers_kth_amd as rf
 
+    if args.n4000_long > 0:
+        print(json.dumps(long_run(rf, args.n4000_long)), flush=True)
+        return
     pos, ori = layout(C2["N"], C2["phi"], C2["seed"])
     g, Pp = timed_run(rf, pos, ori, args.steps, args.warmup, **PLAIN)
     log("N=1000 plain FP64: %.2f steps/s" % g["steps_per_sec"])

@@ -89,7 +89,8 @@ hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double
                        hipEvent_t ev1 = nullptr);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // bytes (a multiple of 4, at most 64 KiB) into coherent pinned host memory
-hipError_t launch_to_host(const void* src, void* host_dst, size_t bytes, hipStream_t s);
+hipError_t launch_to_host(const void* src, void* host_dst, size_t bytes, hipStream_t s,
+                          unsigned* host_seq_word = nullptr, unsigned seq = 0);
 hipError_t launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
 
 // Deterministic reductions / vector kernels (rf_linalg.hip).

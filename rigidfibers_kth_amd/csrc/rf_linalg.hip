@@ -314,18 +314,28 @@ __global__ void f64_to_f32_kernel(const double* __restrict__ src, float* __restr
 // A few words into coherent pinned host memory, by one workgroup: the
 // runtime's device-to-host copy ends with a system-scope release that stalls
 // the queue ~15 us, once per GMRES iteration.
-__global__ void to_host_kernel(const unsigned* __restrict__ src, unsigned* dst, int words)
+// With seq_word, the copy ends with `seq` stored there at system scope after
+// every thread's data stores have been fenced at system scope, so a host that
+// reads the expected sequence number reads the data of this copy.
+__global__ void to_host_kernel(const unsigned* __restrict__ src, unsigned* dst, int words, unsigned* seq_word,
+                               unsigned seq)
 {
     for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+    if (seq_word) {
+        __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(seq_word, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 }  // namespace
 
-hipError_t launch_to_host(const void* src, void* host_dst, size_t bytes, hipStream_t s)
+hipError_t launch_to_host(const void* src, void* host_dst, size_t bytes, hipStream_t s, unsigned* host_seq_word,
+                          unsigned seq)
 {
     if (bytes % sizeof(unsigned) != 0 || bytes > 64 * 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(to_host_kernel, dim3(1), dim3(256), 0, s, static_cast<const unsigned*>(src),
-                       static_cast<unsigned*>(host_dst), (int)(bytes / sizeof(unsigned)));
+                       static_cast<unsigned*>(host_dst), (int)(bytes / sizeof(unsigned)), host_seq_word, seq);
     return hipGetLastError();
 }
 

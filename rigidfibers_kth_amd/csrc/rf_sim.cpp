@@ -276,12 +276,29 @@ int rf_sim::gemv_A(const double* x, double* y, const double* b, int mode)
     return RF_OK;
 }
 
+// The copy into h_dots whose sequence number is host_seq has landed: its
+// sequence word is written after its data at system scope.  Normally it has
+// by the time the event or the stream reports completion; otherwise wait for
+// the stream and look again, and report a stale copy as an error.
+int rf_sim::await_host_copy()
+{
+    const unsigned* w = host_seq_word();
+    for (int spin = 0; spin < 1024; ++spin)
+        if (__atomic_load_n(w, __ATOMIC_ACQUIRE) == host_seq) return RF_OK;
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (__atomic_load_n(w, __ATOMIC_ACQUIRE) == host_seq) return RF_OK;
+    rf::set_error("the dot block copied to host memory is stale (sequence word " +
+                  std::to_string(__atomic_load_n(w, __ATOMIC_ACQUIRE)) + ", expected " + std::to_string(host_seq) +
+                  ")");
+    return RF_EHIP;
+}
+
 int rf_sim::fetch_dots(int count)
 {
-    HIP_TRY(launch_to_host(d_dots, h_dots, sizeof(double) * count, stream));
+    HIP_TRY(launch_to_host(d_dots, h_dots, sizeof(double) * count, stream, host_seq_word(), ++host_seq));
     HIP_TRY(hipStreamSynchronize(stream));
     resolve_timed();
-    return RF_OK;
+    return await_host_copy();
 }
 
 // ||v||^2 into h_dots[0]
@@ -417,13 +434,15 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
             // The Hessenberg column goes to the host; the next basis vector is
             // already on the device, so its product is queued before the host
             // waits (it is discarded if this iteration ends the solve).
-            HIP_TRY(launch_to_host(d_dots, h_dots, sizeof(double) * 3 * (m + 2), stream));
+            HIP_TRY(launch_to_host(d_dots, h_dots, sizeof(double) * 3 * (m + 2), stream, host_seq_word(),
+                                   ++host_seq));
             HIP_TRY(hipEventRecord(ev_k0, stream));
             if (k < m && total_before(iterOut, m, k) < maxit) {
                 if ((rc = matvec(k))) return rc;
                 w_ready = true;
             }
             HIP_TRY(hipEventSynchronize(ev_k0));
+            if ((rc = await_host_copy())) return rc;
             for (int j = 0; j < k; ++j) H[j + (size_t)jH * ldh] = h_dots[j] + h_dots[(m + 2) + j];
             const double dnormw = std::sqrt(h_dots[2 * (m + 2)]);
             H[k + (size_t)jH * ldh] = dnormw;

@@ -73,7 +73,10 @@ struct rf_sim {
     double* d_fV = nullptr;  // [outer_m + 1][lda]
     double* d_fZ = nullptr;  // [outer_m][lda]
     double* d_fw = nullptr;  // [lda]
-    double* h_dots = nullptr;  // pinned
+    double* h_dots = nullptr;  // pinned; its last double holds the copy's sequence word
+    unsigned host_seq = 0;     // sequence number of the last copy into h_dots
+    unsigned* host_seq_word() const { return reinterpret_cast<unsigned*>(h_dots + 3 * (restart + 2) + 7); }
+    int await_host_copy();
     double* h_y = nullptr;     // pinned
     // neighbourhoods and preconditioner (allocated on first use)
     int knn = 0;           // neighbours kept per fiber

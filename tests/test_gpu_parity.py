@@ -5,10 +5,13 @@ Every call goes through librigidfibers.so (include/rigidfibers.h).
 Tolerances:
   * fixtures (F16.8 text, rounded at 5e-9): |gpu - ref| <= 5e-9 + 2e-12;
   * matrix / rhs / velocities vs oracle: <= 1e-12 of the largest entry for the
-    numeric quadrature; 2e-10 for the analytic inner integrals, whose
-    recurrences (Int_analytical.f90:123-206) amplify last-bit differences:
-    the oracle itself built with and without FMA contraction differs by
-    4e-11 of the matrix scale on the N=21 layout used here;
+    numeric quadrature.  With the analytic inner integrals: matrix 1e-10,
+    rhs 1e-11, velocities 1e-12.  The analytic moment recurrences
+    (Int_analytical.f90:123-206) amplify last-bit differences: on the N=21
+    layouts used here the FP64 oracle is itself 2.3e-11 (M=5) and 3.8e-11
+    (M=8) of the matrix scale from the same code in 80-bit extended
+    precision, and the GPU is 3.5e-11 / 2.6e-11 from the oracle (rhs
+    3.5e-12, velocities 1.6e-13);
   * positions and orientations after K steps: <= 1e-9 relative (2-norm).
 """
 import os
@@ -95,9 +98,11 @@ def test_matrix_rhs_vs_oracle(analytic, M, noqi):
     Ao = oracle.assemble_matrix(analytic, EPS, pos, ori, M, noqi)
     bo = oracle.assemble_rhs(analytic, EPS, pos, ori, np.tile([0, 0, -1.0], 21), M, noqi)
     scale = np.abs(Ao - np.eye(Ao.shape[0])).max()
-    tol = 2e-10 if analytic else 1e-12
-    assert np.abs(A - Ao).max() <= tol * scale
-    assert np.abs(b - bo).max() <= tol * np.abs(bo).max()
+    tol_a, tol_b = (1e-10, 1e-11) if analytic else (1e-12, 1e-12)
+    print("parity A %.2e b %.2e (analytic %d, M %d, NoQI %d)" % (
+        np.abs(A - Ao).max() / scale, np.abs(b - bo).max() / np.abs(bo).max(), analytic, M, noqi))
+    assert np.abs(A - Ao).max() <= tol_a * scale
+    assert np.abs(b - bo).max() <= tol_b * np.abs(bo).max()
 
 
 @pytest.mark.parametrize("analytic", [1, 0])
@@ -113,7 +118,9 @@ def test_velocities_vs_oracle(analytic):
         sim.update_velocities()
         v, w = sim.velocities()
     vo, wo = oracle.compute_velocities(analytic, EPS, pos, ori, force, M, noqi, x)
-    tol = 2e-10 if analytic else 1e-12
+    tol = 1e-12
+    print("parity V %.2e Omega %.2e (analytic %d)" % (np.abs(v - vo).max() / np.abs(vo).max(),
+                                                     np.abs(w - wo).max() / np.abs(wo).max(), analytic))
     assert np.abs(v - vo).max() <= tol * np.abs(vo).max()
     assert np.abs(w - wo).max() <= tol * max(np.abs(wo).max(), 1e-300)
 
@@ -139,7 +146,9 @@ def test_shear_velocities_vs_oracle(analytic, precision):
     vo, wo = oracle.compute_velocities(analytic, EPS, P, T, force, M, noqi, x, sh=sh)
     v0, w0 = oracle.compute_velocities(analytic, EPS, P, T, force, M, noqi, x, sh=0.0)
     assert np.abs(v0 - vo).max() > 0.1 * np.abs(vo).max()  # the shear term is not negligible here
-    tol = 2e-10 if analytic else 1e-12
+    tol = 1e-12
+    print("parity shear V %.2e Omega %.2e (analytic %d, precision %d)" % (
+        np.abs(v - vo).max() / np.abs(vo).max(), np.abs(w - wo).max() / np.abs(wo).max(), analytic, precision))
     assert np.abs(v - vo).max() <= tol * np.abs(vo).max()
     assert np.abs(w - wo).max() <= tol * np.abs(wo).max()
 
@@ -332,21 +341,26 @@ def test_full_size_c2_properties():
     scale = np.abs(A - np.eye(A.shape[0])).max()
     for i in rng.choice(N, 8, replace=False):
         assert np.array_equal(A[i * m:(i + 1) * m, i * m:(i + 1) * m], np.eye(m))
+    worst = 0.0
     for _ in range(24):
         i, j = rng.choice(N, 2, replace=False)
         blk = oracle.matrix_block(1, EPS, P3[i], T3[i], P3[j], T3[j], M, noqi)
         mine = A[i * m:(i + 1) * m, j * m:(j + 1) * m]
-        assert np.abs(mine - blk).max() <= 2e-10 * scale
+        worst = max(worst, np.abs(mine - blk).max() / scale)
+        assert np.abs(mine - blk).max() <= 1e-10 * scale
     targets = rng.choice(N, 6, replace=False)
     force = np.tile([0, 0, -1.0], N)
     bo = oracle.assemble_rhs_rows(1, EPS, pos, ori, force, M, noqi, targets)
     bm = np.concatenate([b[t * m:(t + 1) * m] for t in targets])
-    assert np.abs(bm - bo).max() <= 2e-10 * np.abs(bo).max()
     vo, wo = oracle.compute_velocities_rows(1, EPS, pos, ori, force, M, noqi, x, targets)
     vm = np.concatenate([v[3 * t:3 * t + 3] for t in targets])
     wm = np.concatenate([w[3 * t:3 * t + 3] for t in targets])
-    assert np.abs(vm - vo).max() <= 2e-10 * np.abs(vo).max()
-    assert np.abs(wm - wo).max() <= 2e-10 * np.abs(wo).max()
+    print("parity N=1000 blocks %.2e rhs %.2e V %.2e Omega %.2e" % (
+        worst, np.abs(bm - bo).max() / np.abs(bo).max(), np.abs(vm - vo).max() / np.abs(vo).max(),
+        np.abs(wm - wo).max() / np.abs(wo).max()))
+    assert np.abs(bm - bo).max() <= 1e-11 * np.abs(bo).max()
+    assert np.abs(vm - vo).max() <= 1e-12 * np.abs(vo).max()
+    assert np.abs(wm - wo).max() <= 1e-12 * np.abs(wo).max()
 
 
 # Blocks of the FP32 matrix (far-field moments from FP32 backward recurrences,
@@ -369,6 +383,8 @@ def _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, tol, rng, pairs=16, rows=4,
         blk = oracle.matrix_block(1, EPS, P3[i], T3[i], P3[j], T3[j], M, noqi)
         blocks.append((sim.matrix_block(i * m, j * m, m, m), blk))
     scale = max(np.abs(o).max() for _, o in blocks)
+    print("spot checks: blocks %.2e of the largest sampled entry" % (
+        max(np.abs(mine - blk).max() for mine, blk in blocks) / scale))
     for mine, blk in blocks:
         assert np.abs(mine - blk).max() <= tol * scale
     for i in rng.choice(N, 4, replace=False):
@@ -382,6 +398,9 @@ def _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, tol, rng, pairs=16, rows=4,
     vo, wo = oracle.compute_velocities_rows(1, EPS, pos, ori, force, M, noqi, x, targets)
     vm = np.concatenate([v[3 * t:3 * t + 3] for t in targets])
     wm = np.concatenate([w[3 * t:3 * t + 3] for t in targets])
+    print("spot checks: rhs %.2e, V %.2e, Omega %.2e" % (np.abs(bm - bo).max() / np.abs(bo).max(),
+                                                       np.abs(vm - vo).max() / np.abs(vo).max(),
+                                                       np.abs(wm - wo).max() / np.abs(wo).max()))
     assert np.abs(vm - vo).max() <= tv * np.abs(vo).max()
     assert np.abs(wm - wo).max() <= tv * np.abs(wo).max()
 
@@ -414,9 +433,9 @@ def _fp64_residual_checks(sim, pos, ori, x, M, noqi, tol, rng, rows=3):
     ro = bo - yo
     rg = np.concatenate([r[t * m:(t + 1) * m] for t in targets])
     bg = np.concatenate([b64[t * m:(t + 1) * m] for t in targets])
-    print("FP64 backward error %.2e; oracle rows: |r| %.2e of |b| (GPU rows %.2e); |r_gpu - r_oracle| %.2e of |b|" % (
-        be, np.linalg.norm(ro) / np.linalg.norm(bo), np.linalg.norm(rg) / np.linalg.norm(bg),
-        np.abs(rg - ro).max() / np.abs(bo).max()))
+    print("FP64 backward error %.2e; oracle rows: |r| %.2e of |b| (GPU rows %.2e); |r_gpu - r_oracle| %.2e of |b|; "
+          "rhs rows %.2e" % (be, np.linalg.norm(ro) / np.linalg.norm(bo), np.linalg.norm(rg) / np.linalg.norm(bg),
+                             np.abs(rg - ro).max() / np.abs(bo).max(), np.abs(bg - bo).max() / np.abs(bo).max()))
     if st.matrix_elem_bytes == 8:
         noise = np.linalg.norm(sim.operator(rf.OP_AX, x) - sim.matvec(x)) / np.linalg.norm(b64)
         print("operator vs stored FP64 matrix on x: %.2e of |b|" % noise)
@@ -426,7 +445,7 @@ def _fp64_residual_checks(sim, pos, ori, x, M, noqi, tol, rng, rows=3):
         assert be <= 1.01 * 16 * st.last_operator_noise
     else:
         assert be <= 1.01 * tol
-    assert np.abs(bg - bo).max() <= 2e-10 * np.abs(bo).max()
+    assert np.abs(bg - bo).max() <= 1e-11 * np.abs(bo).max()  # 1.7e-12 measured
     # the oracle's own rows of the FP64 residual: at the solver's tolerance
     # (an FP32-operator solution leaves ~1e-7 here)
     assert np.linalg.norm(ro) <= 20 * tol * np.linalg.norm(bo)
@@ -479,7 +498,8 @@ def test_full_size_n4000_properties():
         assert st.last_backward_error <= 1e-10
         assert np.linalg.norm(r) <= 1.01e-10 * np.linalg.norm(b)
         _fp64_residual_checks(sim, pos, ori, x, M, noqi, 1e-10, np.random.default_rng(3))
-        _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, 2e-10, np.random.default_rng(4))
+        # measured: blocks 2.4e-11, rhs 5.6e-14, V 4.1e-14, Omega 3.4e-12
+        _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, 1e-10, np.random.default_rng(4), tol_vel=1e-11)
 
 
 def test_full_size_n4000_mixed_learned():
@@ -516,7 +536,7 @@ def test_full_size_n4000_mixed_learned():
             assert _meets_tol(st, 1e-10)
             _fp64_residual_checks(sim, P, T, x, M, noqi, 1e-10, np.random.default_rng(40 + k))
             _spot_checks(sim, P, T, x, v, w, b, M, noqi, FP32_BLOCK_TOL, np.random.default_rng(50 + k), pairs=8,
-                         tol_vel=2e-10)
+                         tol_vel=5e-11)
             sim.update_fibers()
 
 
@@ -549,7 +569,7 @@ def test_full_size_n10000_mixed_steps():
             _fp64_residual_checks(sim, P, T, x, M, noqi, 1e-8, np.random.default_rng(60 + k), rows=2)
             if k == 0:
                 _spot_checks(sim, P, T, x, v, w, b, M, noqi, FP32_BLOCK_TOL, np.random.default_rng(5), pairs=8,
-                             tol_vel=2e-10)
+                             tol_vel=5e-11)
             sim.update_fibers()
             hist.append(sim.state())
     X0, X1, X2 = hist[0][0], hist[1][0], hist[2][0]
@@ -580,4 +600,4 @@ def test_full_size_n2000_mixed_lu_properties():
         assert st.last_solver == rf.SOLVER_LU and st.lu_factorizations == 1
         assert st.last_backward_error <= 1e-10
         _fp64_residual_checks(sim, pos, ori, x, M, noqi, 1e-10, np.random.default_rng(7))
-        _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, FP32_BLOCK_TOL, np.random.default_rng(6), pairs=12, tol_vel=2e-10)
+        _spot_checks(sim, pos, ori, x, v, w, b, M, noqi, FP32_BLOCK_TOL, np.random.default_rng(6), pairs=12, tol_vel=5e-11)

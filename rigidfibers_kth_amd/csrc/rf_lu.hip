@@ -1249,6 +1249,196 @@ __global__ void __launch_bounds__(256, 3) lu_gemm_f16p_kernel(float* A, int64_t 
     f16_update_tile(A, lda, K1 + st.ri, K1 + st.cj, n, n, vec, kwp / kHK, st, colscale, As, Bs);
 }
 
+// The same update with 256 x 256 tiles, one workgroup per CU.  Each
+// wavefront owns a 128 x 128 quadrant (8 x 8 blocks of 16 x 16, 256 FP32
+// accumulators per lane in the accumulation registers): 64 flops per LDS
+// byte read, twice the 64 x 64 quadrants' 32.  The K chunks travel from the
+// packed operands straight into a ring of kWStages LDS stages
+// (global_load_lds_dwordx4: no staging registers), issued kWStages - 1 chunks
+// ahead, so one wavefront per SIMD still has three chunks of latency cover.
+// Stage rows are 64 bytes with no padding; the 16-byte segment s of row r
+// sits at s ^ f(r), f(r) = ((r >> 1) ^ (r >> 3)) & 3, so the fragment reads
+// (16 rows, one segment) touch every bank once per 8- or 16-lane group; each
+// load instruction fills 1 KB of a stage contiguously (16 rows), its lanes
+// fetching the segments that the swizzle puts in their slots.
+// Measured on the first update of n = 60000 (59000^2 x 1024): the K loop
+// alone runs at 1.8 PFLOP/s, 73 % of the dense fp16 peak; with the FP32 C
+// tile's read-modify-write (8 bytes per element and super-panel, 28 GB here)
+// the update takes 12.6 ms, 565 TFLOP/s: one workgroup per CU has no other
+// workgroup to overlap that traffic with.
+constexpr int kWB = 256, kWStages = 4;
+constexpr int kXcds = 8, kWBlockRows = 4, kWBlockCols = 8;  // tiles per XCD round: one per CU of the XCD
+// below this trailing size the 128 x 128 tiles fill the chip better
+constexpr int kWideMin = 4096;
+constexpr int kWStageHalves = kWB * kHK;  // one operand's chunk: 16 KB
+__device__ __forceinline__ int wide_seg(int r, int s) { return (s ^ ((r >> 1) ^ (r >> 3))) & 3; }
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// Barrier without the memory fence of __syncthreads(): the stage data
+// arrives by the loads' own counter (waited for explicitly before it), and
+// the clobber keeps the compiler from moving LDS reads across it.
+__device__ __forceinline__ void stage_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+// Wait until at most two chunks' loads (8 per wavefront each) are in
+// flight.  s_waitcnt immediate: vmcnt in bits 3:0 and 15:14 (16 = 0x4000),
+// expcnt in 6:4 (7: no wait), lgkmcnt in 11:8 (15: no wait).
+__device__ __forceinline__ void wait_two_chunks_ahead() { __builtin_amdgcn_s_waitcnt(0x4000 | 0x0F70); }
+// ... or two chunks and the 32 C loads issued after the first (vmcnt 48 = 0x8000)
+__device__ __forceinline__ void wait_two_chunks_and_c_ahead() { __builtin_amdgcn_s_waitcnt(0x8000 | 0x0F70); }
+
+__global__ void __launch_bounds__(256, 1) lu_gemm_f16w_kernel(float* A, int64_t lda, int n, int K1, int kwp,
+                                                          const _Float16* __restrict__ Lp,
+                                                          const _Float16* __restrict__ Up,
+                                                          const float* __restrict__ colscale)
+{
+    static_assert(kWStages == 4, "the waits count the chunks issued ahead of the one read");
+    __shared__ __attribute__((aligned(1024))) _Float16 ring[kWStages][2][kWStageHalves];
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wr = (wave >> 1) * 128, wc = (wave & 1) * 128;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int rn = n - K1;
+    // Tile of this workgroup: workgroup ids go round-robin over the 8 XCDs,
+    // and the 32 consecutive workgroups one XCD runs together take a 4 x 8
+    // block of tiles, so its L2 serves each chunk of 4 L panels and 8 U
+    // panels to all 32 (a row-major order would stream a different L panel
+    // per workgroup).  Ids past the last tile leave before the first barrier.
+    const int T = (rn + kWB - 1) / kWB;
+    const int nbi = (T + kWBlockRows - 1) / kWBlockRows, nbj = (T + kWBlockCols - 1) / kWBlockCols;
+    const int id = blockIdx.x, xcd = id % kXcds, local = id / kXcds;
+    const int blk = (local / (kWBlockRows * kWBlockCols)) * kXcds + xcd, slot = local % (kWBlockRows * kWBlockCols);
+    const int tx = (blk % nbi) * kWBlockRows + slot % kWBlockRows;
+    const int ty = (blk / nbi) * kWBlockCols + slot / kWBlockRows;
+    if (blk >= nbi * nbj || tx >= T || ty >= T) return;
+    const int ri = tx * kWB, cj = ty * kWB;  // packed row / column of the tile
+    const int rbase = K1 + ri, cbase = K1 + cj;
+    const bool vec = (K1 % 4 == 0) && (lda % 4 == 0);
+    const int nchunks = kwp / kHK;
+    // this lane's source rows for the wavefront's four 16-row groups of each
+    // operand (rows past the trailing matrix are clamped to its last row:
+    // their products land only in rows / columns that are never stored)
+    const _Float16* asrc[4];
+    const _Float16* bsrc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = wave * 64 + q * 16 + (lane >> 2);
+        const int seg = wide_seg(r, lane & 3) * 8;
+        asrc[q] = Lp + (size_t)min(ri + r, rn - 1) * kPackK + seg;
+        bsrc[q] = Up + (size_t)min(cj + r, rn - 1) * kPackK + seg;
+    }
+    auto issue = [&](int chunk, int st) {
+        const int kc = chunk * kHK;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int off = (wave * 64 + q * 16) * kHK;
+            __builtin_amdgcn_global_load_lds(asrc[q] + kc, (lds_ptr_t)&ring[st][0][off], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(bsrc[q] + kc, (lds_ptr_t)&ring[st][1][off], 16, 0, 0);
+        }
+    };
+    // Every iteration issues one chunk, kWStages - 1 ahead of the one it
+    // reads; past the last chunk the last one is loaded again into the stage
+    // just read (no later iteration reads it), so exactly two chunks' loads
+    // follow the one waited for and the loop has no branches.
+    for (int c = 0; c < kWStages - 1; ++c) issue(min(c, nchunks - 1), c);
+    f4 acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    // this lane's 16 bytes of rows r0..r0+15 of operand o (0: L, 1: U) in a stage
+    auto frag = [&](int st, int o, int r0) {
+        const int r = r0 + lr;
+        return *reinterpret_cast<const h8*>(&ring[st][o][r * kHK + wide_seg(r, lg) * 8]);
+    };
+    // Interior tiles read the first half of their C tile (columns 0..63 of
+    // each quadrant) during the K loop: 32 loads issued in the first
+    // iteration after its chunk loads, consumed after the loop.  They are
+    // younger than chunks 1..3, so those iterations allow 48 loads in flight.
+    const bool interior = vec && rbase + kWB <= n && cbase + kWB <= n;
+    f4 cpre[4][8];
+    for (int ch = 0; ch < nchunks; ++ch) {
+        if (interior && ch >= 1 && ch <= 3)
+            wait_two_chunks_and_c_ahead();
+        else
+            wait_two_chunks_ahead();  // this wavefront's loads of chunk ch have landed
+        stage_barrier();  // ... and every wavefront's; stage (ch - 1) % kWStages has no readers left
+        const int st = ch % kWStages;
+        issue(min(ch + kWStages - 1, nchunks - 1), (ch + kWStages - 1) % kWStages);
+        if (ch == 0 && interior) {
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    cpre[jj][i] = *reinterpret_cast<const f4*>(A + (int64_t)(cbase + wc + 16 * jj + lr) * lda + rbase +
+                                                               wr + 16 * i + 4 * lg);
+        }
+        h8 a[8], b[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a[i] = frag(st, 0, wr + 16 * i);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[j] = frag(st, 1, wc + 16 * j);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+        // the next loads as early as possible, the 16 fragment reads, the 64 products
+        __builtin_amdgcn_sched_group_barrier(0x020, 8, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 64, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // no load into LDS outlives the workgroup
+    // C += (-L)(U s) / s.  Interior tiles: no per-element conditions; the
+    // first half of C arrived during the loop, the second half's 32 loads
+    // per lane are in flight at once; edge tiles are guarded.
+    float inv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int col = cbase + wc + 16 * j + lr;
+        inv[j] = (col < n) ? 1.f / colscale[col] : 0.f;
+    }
+    if (interior) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            __builtin_amdgcn_sched_barrier(0);
+            f4 c[4][8];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int j = 4 * h + jj;
+                    c[jj][i] = (h == 0) ? cpre[jj][i]
+                                        : *reinterpret_cast<const f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda +
+                                                                       rbase + wr + 16 * i + 4 * lg);
+                }
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int j = 4 * h + jj;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) c[jj][i][r] = fmaf(acc[i][j][r], inv[j], c[jj][i][r]);
+                    *reinterpret_cast<f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda + rbase + wr + 16 * i +
+                                           4 * lg) = c[jj][i];
+                }
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int col = cbase + wc + 16 * j + lr;
+        if (col >= n) continue;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row0 = rbase + wr + 16 * i + 4 * lg;
+            float* p = A + (int64_t)col * lda + row0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (row0 + r < n) p[r] = fmaf(acc[i][j][r], inv[j], p[r]);
+        }
+    }
+}
+
 // Block-pivoted panel [k0, k1): diagonal block, then L21 below it; bak
 // receives the panel's original columns (rows [k0, n), leading dimension n).
 template <typename T>
@@ -1378,9 +1568,17 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
                 const int64_t units = (int64_t)rem * (kwp / 8);
                 hipLaunchKernelGGL(pack_u_kernel, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, s, A, lda, n,
                                    K0, K1, kwp, colscale, upack);
-                const int tiles = (rem + kHB - 1) / kHB;
-                hipLaunchKernelGGL(lu_gemm_f16p_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, lda, n, K1, kwp, lpack,
-                                   upack, colscale);
+                if (rem >= kWideMin) {  // 256 x 256 tiles, one workgroup per CU, in XCD-local blocks
+                    const int nt = (rem + kWB - 1) / kWB;
+                    const int blocks = ((nt + kWBlockRows - 1) / kWBlockRows) * ((nt + kWBlockCols - 1) / kWBlockCols);
+                    const int groups = (blocks + kXcds - 1) / kXcds * kXcds * kWBlockRows * kWBlockCols;
+                    hipLaunchKernelGGL(lu_gemm_f16w_kernel, dim3(groups), dim3(256), 0, s, A, lda, n, K1, kwp,
+                                       lpack, upack, colscale);
+                } else {
+                    const int tiles = (rem + kHB - 1) / kHB;
+                    hipLaunchKernelGGL(lu_gemm_f16p_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, lda, n, K1, kwp,
+                                       lpack, upack, colscale);
+                }
             }
         } else {
             gemm({K1, n, K1, n, K0, K1});  // the trailing matrix

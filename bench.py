@@ -192,7 +192,7 @@ TRAIN_SOLVE = dict(gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-10, prec
                    lu_update_fp16=1)
 
 
-def train_network(rf, N, phi, seeds, states, held_pos, held_ori, epochs=40):
+def train_network(rf, N, phi, seeds, states, held_pos, held_ori, epochs=40, held_kw=TRAIN_SOLVE, solve_kw=TRAIN_SOLVE):
     """Solved (configuration, coefficients) pairs from `states` steps of each
     seeded layout (N fibers, volume fraction phi), GPU training of the
     network, and its loss on the pairs of a held-out configuration (the
@@ -202,7 +202,7 @@ def train_network(rf, N, phi, seeds, states, held_pos, held_ori, epochs=40):
     t0 = time.perf_counter()
     for sd in seeds:
         pos, ori = layout(N, phi, sd)
-        sim = rf.Simulation(rf.Configuration(rf.default_params(**TRAIN_SOLVE), pos, ori))
+        sim = rf.Simulation(rf.Configuration(rf.default_params(**solve_kw), pos, ori))
         for _ in range(states):
             sim.assemble_system()
             sim.solve_system()
@@ -214,7 +214,7 @@ def train_network(rf, N, phi, seeds, states, held_pos, held_ori, epochs=40):
         sim.close()
     X, Y = np.concatenate(Xs), np.concatenate(Ys)
     t1 = time.perf_counter()
-    sim = rf.Simulation(rf.Configuration(rf.default_params(**TRAIN_SOLVE), held_pos, held_ori))
+    sim = rf.Simulation(rf.Configuration(rf.default_params(**held_kw), held_pos, held_ori))
     sim.assemble_system()
     sim.solve_system()
     Xh, Yh = sim.mlp_samples()
@@ -318,47 +318,113 @@ def compare(rf, cfg, steps, warmup, train, label, plain_ref):
     return out
 
 
-def long_run(rf, steps):
-    """`--n4000-long K`: K steps at N=4000 from the seeded layout with the
-    plain FP64 direct solver, the mixed + learned path and the 1e-13 FP64
-    reference; steps/s of the first two and the relative position
-    differences at checkpoints (progress on stderr every 10 steps)."""
-    P0, T0 = layout(C4["N"], C4["phi"], C4["seed"])
-    w, tinfo = train_network(rf, 2000, C4["phi"], [C4["seed"] + 0x100 + k for k in range(5)], 2, P0, T0)
+# the long runs (`--long c3|c4|c5 K`): the reference configurations over
+# their full step counts
+C3 = dict(N=2000, phi=2e-2, seed=0x5EED0002)
+C5 = dict(N=10000, phi=1e-2, seed=0x5EED0004)
+# N=10000: FP32 matrix (90 GB) with the neighbour block-Jacobi preconditioner
+# (an FP32 LU copy would not fit beside it), flexible GMRES on the FP64
+# operator to 1e-8, learned guess, dt=0.05
+C5_MIXED = dict(MIXED, gmres_tol=1e-8, precond=1, timestep=0.05)
+LONG = {
+    "c3": dict(cfg=C3, steps=50,
+               label="N=2000, volume fraction 2e-2, M=5, dt=0.1, tol 1e-10, from the seeded layout (seed 0x5EED0002)",
+               methods=(("mixed", MIXED_NO_MLP), ("plain_fp64", PLAIN), ("plain_fp64_direct", PLAIN_DIRECT),
+                        ("tight_fp64", TIGHT)),
+               train=None, shadow_ref="tight_fp64"),
+    "c4": dict(cfg=C4, steps=100,
+               label="N=4000, volume fraction 5e-2, M=5, dt=0.1, tol 1e-10, from the seeded layout (seed 0x5EED0003)",
+               methods=(("mixed_learned", MIXED), ("plain_fp64_direct", PLAIN_DIRECT), ("tight_fp64", TIGHT)),
+               train=dict(N=2000, seeds=[C4["seed"] + 0x100 + k for k in range(5)], held=TRAIN_SOLVE),
+               shadow_ref="tight_fp64"),
+    "c5": dict(cfg=C5, steps=100,
+               label="N=10000, volume fraction 1e-2, M=5, dt=0.05, tol 1e-8, from the seeded layout (seed 0x5EED0004)",
+               methods=(("mixed_learned", C5_MIXED),),
+               # solved pairs of two other N=10000 layouts (the FP64 matrix and
+               # its FP32 factors would not fit beside each other at this size;
+               # the mixed path solves them to 1e-10 against the FP64 operator)
+               train=dict(N=10000, seeds=[C5["seed"] + 0x100 + k for k in range(2)],
+                          held=dict(C5_MIXED, warm_start=0, gmres_tol=1e-10),
+                          solve=dict(C5_MIXED, warm_start=0, gmres_tol=1e-10)),
+               shadow_ref=None),
+}
+
+
+def fiber_spread(a, b, bound=ACCURACY):
+    """Relative position difference of two runs, overall and per fiber (each
+    fiber's displacement difference over the rms fiber position)."""
+    d = np.linalg.norm((a - b).reshape(-1, 3), axis=1)
+    scale = np.linalg.norm(b) / np.sqrt(d.size)
+    r = d / scale
+    return {"rel": rel(a, b), "fiber_median": float(np.median(r)), "fiber_p90": float(np.percentile(r, 90)),
+            "fiber_max": float(r.max()), "fibers_within_bound": float(np.mean(r <= bound))}
+
+
+def long_run(rf, name, steps):
+    """`--long c3|c4|c5 K`: K steps of each method of the configuration from
+    its seeded layout: steps/s, LU factorisations, the largest FP64 backward
+    error of any step, positions compared at checkpoints (overall and per
+    fiber), and the per-step shadow error of the first method against the
+    reference method over all K steps (progress on stderr every 10 steps)."""
+    spec = LONG[name]
+    cfg = spec["cfg"]
+    P0, T0 = layout(cfg["N"], cfg["phi"], cfg["seed"])
+    w, tinfo = None, None
+    if spec["train"]:
+        tr = spec["train"]
+        w, tinfo = train_network(rf, tr["N"], cfg["phi"], tr["seeds"], 2, P0, T0, held_kw=tr["held"],
+                                 solve_kw=tr.get("solve", TRAIN_SOLVE))
+        log("%s: network trained on %d pairs, held-out loss %.3f" % (name, tinfo["pairs"], tinfo["held_out_loss"]))
     marks = sorted({k for k in (1, 2, 3, 5, 10, 20, 30, 50, 70, 100, steps) if k <= steps})
     runs = {}
-    for name, kw, wts in (("mixed_learned", MIXED, w), ("plain_fp64_direct", PLAIN_DIRECT, None),
-                          ("tight_fp64", TIGHT, None)):
+    for mname, kw in spec["methods"]:
         sim = rf.Simulation(rf.Configuration(rf.default_params(profile=1, **kw), P0, T0))
-        if wts is not None:
-            sim.mlp_set_weights(wts)
-        snaps, t0 = {}, time.perf_counter()
+        if kw.get("warm_start") == 3:
+            sim.mlp_set_weights(w)
+        snaps, worst_be, noise_limited, t0 = {}, 0.0, 0, time.perf_counter()
         for k in range(1, steps + 1):
             sim.step()
+            st = sim.stats()  # host copy of counters: no wait on the stream
+            worst_be = max(worst_be, float(st.last_backward_error))
+            noise_limited += int(st.last_noise_limited)
             if k in marks:
                 snaps[k] = sim.state()[0]
             if k % 10 == 0:
-                log("%s: %d steps, %.1f s" % (name, k, time.perf_counter() - t0))
+                log("%s %s: %d steps, %.1f s" % (name, mname, k, time.perf_counter() - t0))
         sim.synchronize()
         wall = time.perf_counter() - t0
         st = sim.stats()
         sim.close()
-        runs[name] = {"snaps": snaps, "steps_per_sec": steps / wall, "lu_factorizations": st.lu_factorizations,
-                      "gmres_iters": st.total_gmres_iters}
+        runs[mname] = {"snaps": snaps, "steps_per_sec": steps / wall, "lu_factorizations": int(st.lu_factorizations),
+                       "gmres_iters": int(st.total_gmres_iters), "max_backward_error": worst_be,
+                       "noise_limited_steps": noise_limited}
+    names = [m for m, _ in spec["methods"]]
     diffs = {}
-    for a, b in (("mixed_learned", "plain_fp64_direct"), ("plain_fp64_direct", "tight_fp64"),
-                 ("mixed_learned", "tight_fp64")):
-        diffs["%s_vs_%s" % (a, b)] = {str(k): rel(runs[a]["snaps"][k], runs[b]["snaps"][k]) for k in marks}
-    return {
-        "workload": "N=4000, volume fraction 5e-2, M=5, dt=0.1, %d steps from the seeded layout (seed 0x%X)"
-                    % (steps, C4["seed"]),
+    for i, a in enumerate(names):
+        for b in names[i + 1:]:
+            diffs["%s_vs_%s" % (a, b)] = {str(k): fiber_spread(runs[a]["snaps"][k], runs[b]["snaps"][k])
+                                          for k in marks}
+    out = {
+        "workload": spec["label"] + ", %d steps" % steps,
         "steps": steps,
         "training": tinfo,
-        "steps_per_sec": {k: v["steps_per_sec"] for k, v in runs.items()},
-        "speedup_vs_plain_direct": runs["mixed_learned"]["steps_per_sec"] / runs["plain_fp64_direct"]["steps_per_sec"],
-        "lu_factorizations": {k: v["lu_factorizations"] for k, v in runs.items()},
-        "position_rel_diff_at_step": diffs,
+        "methods": {m: {k: v for k, v in r.items() if k != "snaps"} for m, r in runs.items()},
+        "position_diff_at_step": diffs,
+        "position_diff_unit": "rel: |X_a - X_b| / |X_b| over all fibers; fiber_*: one fiber's displacement "
+                              "difference over the rms fiber position; fibers_within_bound: share <= 1e-6",
     }
+    plain = [m for m in names if m.startswith("plain")]
+    if plain:
+        best = max(runs[m]["steps_per_sec"] for m in plain)
+        out["speedup_vs_best_plain"] = runs[names[0]]["steps_per_sec"] / best
+    if spec["shadow_ref"]:
+        ref_kw = dict(spec["methods"])[spec["shadow_ref"]]
+        sh = shadow(rf, P0, T0, steps, 0, w, spec["methods"][0][1], {"ref": ref_kw})
+        per = sh["method_vs_ref"]
+        out["shadow"] = {"reference": spec["shadow_ref"], "max_rel_err": max(per), "per_step": per,
+                         "meets_accuracy": bool(max(per) <= ACCURACY)}
+        log("%s: shadow error over %d steps, max %.2e" % (name, steps, max(per)))
+    return out
 
 
 def cpu_baseline(N, phi, seed, threads, warmup=2, steps=5):
@@ -406,9 +472,10 @@ def main():
     ap.add_argument("--no-lu", action="store_true", help="skip the direct-solver timing")
     ap.add_argument("--no-n4000", action="store_true", help="skip the N=4000 comparison")
     ap.add_argument("--n4000-steps", type=int, default=3)
-    ap.add_argument("--n4000-long", type=int, default=0, metavar="K",
-                    help="instead of the benchmark: K steps at N=4000 of the plain direct solver, mixed + learned "
-                         "and the 1e-13 reference with position differences at checkpoints (one JSON line)")
+    ap.add_argument("--long", nargs=2, metavar=("CONFIG", "K"),
+                    help="instead of the benchmark: K steps of configuration c3 (N=2000), c4 (N=4000) or c5 "
+                         "(N=10000) for each of its methods, with position differences at checkpoints and the "
+                         "per-step shadow error over all K steps (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the multi-GPU row sharding is a design only "
@@ -416,8 +483,10 @@ def main():
         sys.exit(2)
     import rigidfibers_kth_amd as rf
 
-    if args.n4000_long > 0:
-        print(json.dumps(long_run(rf, args.n4000_long)), flush=True)
+    if args.long:
+        if args.long[0] not in LONG or int(args.long[1]) < 1:
+            ap.error("--long takes c3, c4 or c5 and a positive step count")
+        print(json.dumps(long_run(rf, args.long[0], int(args.long[1]))), flush=True)
         return
     pos, ori = layout(C2["N"], C2["phi"], C2["seed"])
     g, Pp = timed_run(rf, pos, ori, args.steps, args.warmup, **PLAIN)

@@ -1249,173 +1249,204 @@ __global__ void __launch_bounds__(256, 3) lu_gemm_f16p_kernel(float* A, int64_t 
     f16_update_tile(A, lda, K1 + st.ri, K1 + st.cj, n, n, vec, kwp / kHK, st, colscale, As, Bs);
 }
 
-// The same update with 256 x 256 tiles, one workgroup per CU.  Each
-// wavefront owns a 128 x 128 quadrant (8 x 8 blocks of 16 x 16, 256 FP32
-// accumulators per lane in the accumulation registers): 64 flops per LDS
-// byte read, twice the 64 x 64 quadrants' 32.  The K chunks travel from the
-// packed operands straight into a ring of kWStages LDS stages
-// (global_load_lds_dwordx4: no staging registers), issued kWStages - 1 chunks
-// ahead, so one wavefront per SIMD still has three chunks of latency cover.
-// Stage rows are 64 bytes with no padding; the 16-byte segment s of row r
-// sits at s ^ f(r), f(r) = ((r >> 1) ^ (r >> 3)) & 3, so the fragment reads
-// (16 rows, one segment) touch every bank once per 8- or 16-lane group; each
-// load instruction fills 1 KB of a stage contiguously (16 rows), its lanes
-// fetching the segments that the swizzle puts in their slots.
-// Measured on the first update of n = 60000 (59000^2 x 1024): the K loop
-// alone runs at 1.8 PFLOP/s, 73 % of the dense fp16 peak; with the FP32 C
-// tile's read-modify-write (8 bytes per element and super-panel, 28 GB here)
-// the update takes 12.6 ms, 565 TFLOP/s: one workgroup per CU has no other
-// workgroup to overlap that traffic with.
-constexpr int kWB = 256, kWStages = 4;
-constexpr int kXcds = 8, kWBlockRows = 4, kWBlockCols = 8;  // tiles per XCD round: one per CU of the XCD
+// The same update with wide tiles: 256 rows, and each wavefront a 128-row
+// quadrant whose 8 x (TN/32) blocks of 16 x 16 keep their FP32 accumulators
+// in registers for the whole K loop: 43 (TN = 128) or 64 (TN = 256) flops
+// per LDS byte read, against 32 for 64 x 64 quadrants.  The K chunks travel
+// from the packed operands straight into a ring of LDS stages
+// (global_load_lds_dwordx4: no staging registers), issued STAGES - 1 chunks
+// ahead.  Stage rows are 64 bytes with no padding; the 16-byte segment s of
+// row r sits at s ^ f(r), f(r) = ((r >> 1) ^ (r >> 3)) & 3, so the fragment
+// reads (16 rows, one segment) touch every bank once per 8- or 16-lane
+// group; each load instruction fills 1 KB of a stage contiguously (16 rows),
+// its lanes fetching the segments that the swizzle puts in their slots.
+// Measured on the first update of n = 60000 (59000^2 x 1024): with 256 x 256
+// tiles, one workgroup per CU, the K loop alone runs at 1.8 PFLOP/s (73 % of
+// the dense fp16 peak) but the FP32 C tile's read-modify-write (8 bytes per
+// element and super-panel, 28 GB here) does not overlap it: 12.6 ms.  With
+// 256 x 128 tiles two workgroups share a CU and one's C traffic overlaps the
+// other's products: 12.1 ms, and 240 ms against 270 ms over a factorisation.
+constexpr int kWB = 256;  // tile rows
+constexpr int kXcds = 8, kCusPerXcd = 32;
 // below this trailing size the 128 x 128 tiles fill the chip better
 constexpr int kWideMin = 4096;
-constexpr int kWStageHalves = kWB * kHK;  // one operand's chunk: 16 KB
 __device__ __forceinline__ int wide_seg(int r, int s) { return (s ^ ((r >> 1) ^ (r >> 3))) & 3; }
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// 16 bytes per lane from src straight into LDS at dst (wavefront-uniform; lane
+// l's bytes land at dst + 16 l).  Kept out of the kernel templates: the host
+// pass cannot form the LDS address-space pointer.
+__device__ __forceinline__ void ring_load(const _Float16* src, _Float16* dst)
+{
+    __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)dst, 16, 0, 0);
+}
 
 // Barrier without the memory fence of __syncthreads(): the stage data
 // arrives by the loads' own counter (waited for explicitly before it), and
 // the clobber keeps the compiler from moving LDS reads across it.
 __device__ __forceinline__ void stage_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
-// Wait until at most two chunks' loads (8 per wavefront each) are in
-// flight.  s_waitcnt immediate: vmcnt in bits 3:0 and 15:14 (16 = 0x4000),
-// expcnt in 6:4 (7: no wait), lgkmcnt in 11:8 (15: no wait).
-__device__ __forceinline__ void wait_two_chunks_ahead() { __builtin_amdgcn_s_waitcnt(0x4000 | 0x0F70); }
-// ... or two chunks and the 32 C loads issued after the first (vmcnt 48 = 0x8000)
-__device__ __forceinline__ void wait_two_chunks_and_c_ahead() { __builtin_amdgcn_s_waitcnt(0x8000 | 0x0F70); }
+// s_waitcnt immediate for "at most v vector-memory operations in flight":
+// vmcnt in bits 3:0 and 15:14, expcnt in 6:4 (7: no wait), lgkmcnt in 11:8
+// (15: no wait).
+constexpr int vm_wait(int v) { return (v & 15) | ((v >> 4) << 14) | 0x0F70; }
 
-__global__ void __launch_bounds__(256, 1) lu_gemm_f16w_kernel(float* A, int64_t lda, int n, int K1, int kwp,
-                                                          const _Float16* __restrict__ Lp,
-                                                          const _Float16* __restrict__ Up,
-                                                          const float* __restrict__ colscale)
+// Configurations: TN tile columns (each wavefront 128 x TN/2), STAGES ring
+// stages, WG workgroups per CU.  256 x 256 with 4 stages (128 KB of LDS) runs
+// one workgroup per CU and reads half its C tile during the K loop; 256 x 128
+// with 3 stages (72 KB, 128 accumulators per lane) runs two per CU, so one
+// workgroup's C traffic overlaps the other's products.
+template <int TN, int STAGES, int WG>
+struct WideCfg {
+    static constexpr int kBHalves = TN * kHK;               // U operand of one chunk
+    static constexpr int kStageHalves = kWB * kHK + kBHalves;
+    static constexpr int kJB = TN / 32;                      // 16-column blocks per wavefront
+    static constexpr int kLoads = 4 + TN / 64;               // ring loads per wavefront and chunk
+    static constexpr int kBlockRows = (TN == 256) ? 4 : 8;   // tiles per XCD round: WG per CU of the XCD
+    static constexpr int kBlockCols = kCusPerXcd * WG / kBlockRows;
+    static constexpr bool kPreload = (WG == 1);
+};
+using WideBig = WideCfg<256, 4, 1>;
+using WideTwo = WideCfg<128, 3, 2>;
+
+template <int TN, int STAGES, int WG>
+__global__ void __launch_bounds__(256, WG) lu_gemm_f16w_kernel(float* A, int64_t lda, int n, int K1, int kwp,
+                                                           const _Float16* __restrict__ Lp,
+                                                           const _Float16* __restrict__ Up,
+                                                           const float* __restrict__ colscale)
 {
-    static_assert(kWStages == 4, "the waits count the chunks issued ahead of the one read");
-    __shared__ __attribute__((aligned(1024))) _Float16 ring[kWStages][2][kWStageHalves];
+    using C = WideCfg<TN, STAGES, WG>;
+    constexpr int JB = C::kJB;
+    __shared__ __attribute__((aligned(1024))) _Float16 ring[STAGES][C::kStageHalves];
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-    const int wr = (wave >> 1) * 128, wc = (wave & 1) * 128;
+    const int wr = (wave >> 1) * 128, wc = (wave & 1) * (TN / 2);
     const int lr = lane & 15, lg = lane >> 4;
     const int rn = n - K1;
     // Tile of this workgroup: workgroup ids go round-robin over the 8 XCDs,
-    // and the 32 consecutive workgroups one XCD runs together take a 4 x 8
-    // block of tiles, so its L2 serves each chunk of 4 L panels and 8 U
-    // panels to all 32 (a row-major order would stream a different L panel
-    // per workgroup).  Ids past the last tile leave before the first barrier.
-    const int T = (rn + kWB - 1) / kWB;
-    const int nbi = (T + kWBlockRows - 1) / kWBlockRows, nbj = (T + kWBlockCols - 1) / kWBlockCols;
+    // and the workgroups one XCD runs together take a block of tiles, so its
+    // L2 serves each chunk of a few L and U panels to all of them (a
+    // row-major order would stream a different L panel per workgroup).  Ids
+    // past the last tile leave before the first barrier.
+    const int TR = (rn + kWB - 1) / kWB, TC = (rn + TN - 1) / TN;
+    const int nbi = (TR + C::kBlockRows - 1) / C::kBlockRows, nbj = (TC + C::kBlockCols - 1) / C::kBlockCols;
+    constexpr int per = C::kBlockRows * C::kBlockCols;
     const int id = blockIdx.x, xcd = id % kXcds, local = id / kXcds;
-    const int blk = (local / (kWBlockRows * kWBlockCols)) * kXcds + xcd, slot = local % (kWBlockRows * kWBlockCols);
-    const int tx = (blk % nbi) * kWBlockRows + slot % kWBlockRows;
-    const int ty = (blk / nbi) * kWBlockCols + slot / kWBlockRows;
-    if (blk >= nbi * nbj || tx >= T || ty >= T) return;
-    const int ri = tx * kWB, cj = ty * kWB;  // packed row / column of the tile
+    const int blk = (local / per) * kXcds + xcd, slot = local % per;
+    const int tx = (blk % nbi) * C::kBlockRows + slot % C::kBlockRows;
+    const int ty = (blk / nbi) * C::kBlockCols + slot / C::kBlockRows;
+    if (blk >= nbi * nbj || tx >= TR || ty >= TC) return;
+    const int ri = tx * kWB, cj = ty * TN;  // packed row / column of the tile
     const int rbase = K1 + ri, cbase = K1 + cj;
     const bool vec = (K1 % 4 == 0) && (lda % 4 == 0);
     const int nchunks = kwp / kHK;
-    // this lane's source rows for the wavefront's four 16-row groups of each
+    // this lane's source rows for the wavefront's 16-row groups of each
     // operand (rows past the trailing matrix are clamped to its last row:
     // their products land only in rows / columns that are never stored)
     const _Float16* asrc[4];
-    const _Float16* bsrc[4];
+    const _Float16* bsrc[TN / 64];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int r = wave * 64 + q * 16 + (lane >> 2);
-        const int seg = wide_seg(r, lane & 3) * 8;
-        asrc[q] = Lp + (size_t)min(ri + r, rn - 1) * kPackK + seg;
-        bsrc[q] = Up + (size_t)min(cj + r, rn - 1) * kPackK + seg;
+        asrc[q] = Lp + (size_t)min(ri + r, rn - 1) * kPackK + wide_seg(r, lane & 3) * 8;
+    }
+#pragma unroll
+    for (int q = 0; q < TN / 64; ++q) {
+        const int r = wave * (TN / 4) + q * 16 + (lane >> 2);
+        bsrc[q] = Up + (size_t)min(cj + r, rn - 1) * kPackK + wide_seg(r, lane & 3) * 8;
     }
     auto issue = [&](int chunk, int st) {
         const int kc = chunk * kHK;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int off = (wave * 64 + q * 16) * kHK;
-            __builtin_amdgcn_global_load_lds(asrc[q] + kc, (lds_ptr_t)&ring[st][0][off], 16, 0, 0);
-            __builtin_amdgcn_global_load_lds(bsrc[q] + kc, (lds_ptr_t)&ring[st][1][off], 16, 0, 0);
-        }
+        for (int q = 0; q < 4; ++q) ring_load(asrc[q] + kc, &ring[st][(wave * 64 + q * 16) * kHK]);
+#pragma unroll
+        for (int q = 0; q < TN / 64; ++q) ring_load(bsrc[q] + kc, &ring[st][kWB * kHK + (wave * (TN / 4) + q * 16) * kHK]);
     };
-    // Every iteration issues one chunk, kWStages - 1 ahead of the one it
-    // reads; past the last chunk the last one is loaded again into the stage
-    // just read (no later iteration reads it), so exactly two chunks' loads
+    // Every iteration issues one chunk, STAGES - 1 ahead of the one it reads;
+    // past the last chunk the last one is loaded again into the stage just
+    // read (no later iteration reads it), so exactly STAGES - 2 chunks' loads
     // follow the one waited for and the loop has no branches.
-    for (int c = 0; c < kWStages - 1; ++c) issue(min(c, nchunks - 1), c);
-    f4 acc[8][8];
+    for (int c = 0; c < STAGES - 1; ++c) issue(min(c, nchunks - 1), c);
+    f4 acc[8][JB];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < JB; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
     // this lane's 16 bytes of rows r0..r0+15 of operand o (0: L, 1: U) in a stage
     auto frag = [&](int st, int o, int r0) {
         const int r = r0 + lr;
-        return *reinterpret_cast<const h8*>(&ring[st][o][r * kHK + wide_seg(r, lg) * 8]);
+        return *reinterpret_cast<const h8*>(&ring[st][o * kWB * kHK + r * kHK + wide_seg(r, lg) * 8]);
     };
-    // Interior tiles read the first half of their C tile (columns 0..63 of
-    // each quadrant) during the K loop: 32 loads issued in the first
-    // iteration after its chunk loads, consumed after the loop.  They are
-    // younger than chunks 1..3, so those iterations allow 48 loads in flight.
-    const bool interior = vec && rbase + kWB <= n && cbase + kWB <= n;
-    f4 cpre[4][8];
+    // With one workgroup per CU, interior tiles read the first half of their
+    // C tile during the K loop: 32 loads issued in the first iteration after
+    // its chunk loads, consumed after the loop; they are younger than chunks
+    // 1..STAGES-1, so those iterations allow 32 more loads in flight.
+    const bool interior = vec && rbase + kWB <= n && cbase + TN <= n;
+    constexpr int JP = C::kPreload ? JB / 2 : 1;
+    f4 cpre[JP][8];
     for (int ch = 0; ch < nchunks; ++ch) {
-        if (interior && ch >= 1 && ch <= 3)
-            wait_two_chunks_and_c_ahead();
+        if (C::kPreload && interior && ch >= 1 && ch <= STAGES - 1)
+            __builtin_amdgcn_s_waitcnt(vm_wait((STAGES - 2) * C::kLoads + 32));
         else
-            wait_two_chunks_ahead();  // this wavefront's loads of chunk ch have landed
-        stage_barrier();  // ... and every wavefront's; stage (ch - 1) % kWStages has no readers left
-        const int st = ch % kWStages;
-        issue(min(ch + kWStages - 1, nchunks - 1), (ch + kWStages - 1) % kWStages);
-        if (ch == 0 && interior) {
+            __builtin_amdgcn_s_waitcnt(vm_wait((STAGES - 2) * C::kLoads));  // this wavefront's chunk ch has landed
+        stage_barrier();  // ... and every wavefront's; stage (ch - 1) % STAGES has no readers left
+        const int st = ch % STAGES;
+        issue(min(ch + STAGES - 1, nchunks - 1), (ch + STAGES - 1) % STAGES);
+        if (C::kPreload && ch == 0 && interior) {
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
+            for (int jj = 0; jj < JP; ++jj)
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
                     cpre[jj][i] = *reinterpret_cast<const f4*>(A + (int64_t)(cbase + wc + 16 * jj + lr) * lda + rbase +
                                                                wr + 16 * i + 4 * lg);
         }
-        h8 a[8], b[8];
+        h8 a[8], b[JB];
 #pragma unroll
         for (int i = 0; i < 8; ++i) a[i] = frag(st, 0, wr + 16 * i);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) b[j] = frag(st, 1, wc + 16 * j);
+        for (int j = 0; j < JB; ++j) b[j] = frag(st, 1, wc + 16 * j);
 #pragma unroll
         for (int i = 0; i < 8; ++i)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
-        // the next loads as early as possible, the 16 fragment reads, the 64 products
-        __builtin_amdgcn_sched_group_barrier(0x020, 8, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 64, 0);
+            for (int j = 0; j < JB; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+        // the next loads as early as possible, the fragment reads, the products
+        __builtin_amdgcn_sched_group_barrier(0x020, C::kLoads, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 8 + JB, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 8 * JB, 0);
     }
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // no load into LDS outlives the workgroup
-    // C += (-L)(U s) / s.  Interior tiles: no per-element conditions; the
-    // first half of C arrived during the loop, the second half's 32 loads
-    // per lane are in flight at once; edge tiles are guarded.
-    float inv[8];
+    __builtin_amdgcn_s_waitcnt(vm_wait(0));  // no load into LDS outlives the workgroup
+    // C += (-L)(U s) / s.  Interior tiles: no per-element conditions; a
+    // preloaded first half, then the rest with all its loads in flight at
+    // once (at most 32 per lane); edge tiles are guarded.
+    float inv[JB];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < JB; ++j) {
         const int col = cbase + wc + 16 * j + lr;
         inv[j] = (col < n) ? 1.f / colscale[col] : 0.f;
     }
     if (interior) {
+        constexpr int H = C::kPreload ? 2 : 1, JH = JB / H;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < H; ++h) {
             __builtin_amdgcn_sched_barrier(0);
-            f4 c[4][8];
+            f4 c[JH][8];
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
+            for (int jj = 0; jj < JH; ++jj)
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    const int j = 4 * h + jj;
-                    c[jj][i] = (h == 0) ? cpre[jj][i]
-                                        : *reinterpret_cast<const f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda +
-                                                                       rbase + wr + 16 * i + 4 * lg);
+                    const int j = JH * h + jj;
+                    if (C::kPreload && h == 0)
+                        c[jj][i] = cpre[jj % JP][i];
+                    else
+                        c[jj][i] = *reinterpret_cast<const f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda + rbase +
+                                                                wr + 16 * i + 4 * lg);
                 }
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
+            for (int jj = 0; jj < JH; ++jj)
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    const int j = 4 * h + jj;
+                    const int j = JH * h + jj;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) c[jj][i][r] = fmaf(acc[i][j][r], inv[j], c[jj][i][r]);
                     *reinterpret_cast<f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda + rbase + wr + 16 * i +
@@ -1425,7 +1456,7 @@ __global__ void __launch_bounds__(256, 1) lu_gemm_f16w_kernel(float* A, int64_t 
         return;
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < JB; ++j) {
         const int col = cbase + wc + 16 * j + lr;
         if (col >= n) continue;
 #pragma unroll
@@ -1437,6 +1468,19 @@ __global__ void __launch_bounds__(256, 1) lu_gemm_f16w_kernel(float* A, int64_t 
                 if (row0 + r < n) p[r] = fmaf(acc[i][j][r], inv[j], p[r]);
         }
     }
+}
+
+template <class Cfg, int TN, int STAGES, int WG>
+hipError_t launch_wide(float* A, int64_t lda, int n, int K1, int kwp, const _Float16* lpack, const _Float16* upack,
+                       const float* colscale, hipStream_t s)
+{
+    const int rem = n - K1;
+    const int TR = (rem + kWB - 1) / kWB, TC = (rem + TN - 1) / TN;
+    const int blocks = ((TR + Cfg::kBlockRows - 1) / Cfg::kBlockRows) * ((TC + Cfg::kBlockCols - 1) / Cfg::kBlockCols);
+    const int groups = (blocks + kXcds - 1) / kXcds * kXcds * Cfg::kBlockRows * Cfg::kBlockCols;
+    hipLaunchKernelGGL((lu_gemm_f16w_kernel<TN, STAGES, WG>), dim3(groups), dim3(256), 0, s, A, lda, n, K1, kwp, lpack,
+                       upack, colscale);
+    return hipGetLastError();
 }
 
 // Block-pivoted panel [k0, k1): diagonal block, then L21 below it; bak
@@ -1568,12 +1612,9 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
                 const int64_t units = (int64_t)rem * (kwp / 8);
                 hipLaunchKernelGGL(pack_u_kernel, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, s, A, lda, n,
                                    K0, K1, kwp, colscale, upack);
-                if (rem >= kWideMin) {  // 256 x 256 tiles, one workgroup per CU, in XCD-local blocks
-                    const int nt = (rem + kWB - 1) / kWB;
-                    const int blocks = ((nt + kWBlockRows - 1) / kWBlockRows) * ((nt + kWBlockCols - 1) / kWBlockCols);
-                    const int groups = (blocks + kXcds - 1) / kXcds * kXcds * kWBlockRows * kWBlockCols;
-                    hipLaunchKernelGGL(lu_gemm_f16w_kernel, dim3(groups), dim3(256), 0, s, A, lda, n, K1, kwp,
-                                       lpack, upack, colscale);
+                if (rem >= kWideMin) {  // 256 x 128 tiles, two workgroups per CU, in XCD-local blocks
+                    const hipError_t e = launch_wide<WideTwo, 128, 3, 2>(A, lda, n, K1, kwp, lpack, upack, colscale, s);
+                    if (e != hipSuccess) return e;
                 } else {
                     const int tiles = (rem + kHB - 1) / kHB;
                     hipLaunchKernelGGL(lu_gemm_f16p_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, lda, n, K1, kwp,

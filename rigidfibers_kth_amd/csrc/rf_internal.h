@@ -70,6 +70,13 @@ size_t operator_scratch_doubles(const OperatorPlan& p);
 hipError_t launch_operator(const OperatorPlan& p, const Tables& t, int mode, const double* pos, const double* orient,
                            const double* force, const double* x, const double* wp, const double* pv, double* scratch,
                            double* y, hipStream_t s);
+// The output map of `mode` alone, from the split partials the last
+// launch_operator left in `scratch`: every mode's kernel forms all IF_k
+// (k = 0..M), so the velocity sums (mode kOpVelocity) of an evaluation whose
+// source vectors equal the velocity's -- a residual at the same x, with no
+// x/y force component or numeric inner integrals -- need no second pass.
+hipError_t launch_operator_epilogue(const OperatorPlan& p, const Tables& t, int mode, const double* orient,
+                                    const double* x, const double* scratch, double* y, hipStream_t s);
 
 // Column-major GEMV y = A x with FP64 arithmetic; A is double or float.
 // Writes partial sums to parts[splits][rows_padded] then reduces into y.

@@ -327,6 +327,14 @@ hipError_t launch_operator(const OperatorPlan& p, const Tables& t, int mode, con
     case 8: e = launch_op_m<8>(k, grid, p.threads, lds, p.analytic != 0, s); break;
     }
     if (e != hipSuccess) return e;
+    return launch_operator_epilogue(p, t, mode, orient, x, scratch, y, s);
+}
+
+hipError_t launch_operator_epilogue(const OperatorPlan& p, const Tables& t, int mode, const double* orient,
+                                    const double* x, const double* scratch, double* y, hipStream_t s)
+{
+    if (mode < 0 || mode > 3 || p.M != t.M) return hipErrorInvalidValue;
+    const double* part = scratch + (size_t)p.N * p.src_stride;
     OpEpiArgs ea;
     ea.N = p.N;
     ea.M = p.M;

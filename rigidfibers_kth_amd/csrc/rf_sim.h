@@ -31,6 +31,10 @@ struct rf_sim {
     int64_t steps_taken = 0;
     int64_t solves_done = 0;
     bool assembled = false, solved = false, have_velocities = false;
+    // the operator's split partials hold the final FP64 residual of the last
+    // mixed solve at the current x, with source vectors equal to the
+    // velocity's: update_velocities needs only the epilogue
+    bool op_velocity_ready = false;
     double last_bea = 0.0;
 
     double* d_pos[3] = {nullptr, nullptr, nullptr};

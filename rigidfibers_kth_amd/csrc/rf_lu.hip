@@ -1292,10 +1292,9 @@ __device__ __forceinline__ void stage_barrier() { asm volatile("s_barrier" ::: "
 // (15: no wait).
 constexpr int vm_wait(int v) { return (v & 15) | ((v >> 4) << 14) | 0x0F70; }
 
-// Configurations: TN tile columns (each wavefront 128 x TN/2), STAGES ring
-// stages, WG workgroups per CU.  256 x 256 with 4 stages (128 KB of LDS) runs
-// one workgroup per CU and reads half its C tile during the K loop; 256 x 128
-// with 3 stages (72 KB, 128 accumulators per lane) runs two per CU, so one
+// Configuration: TN tile columns (each wavefront 128 x TN/2), STAGES ring
+// stages, WG workgroups per CU.  The one used, 256 x 128 with 3 stages (72 KB
+// of LDS, 128 accumulators per lane), runs two workgroups per CU, so one
 // workgroup's C traffic overlaps the other's products.
 template <int TN, int STAGES, int WG>
 struct WideCfg {
@@ -1305,9 +1304,7 @@ struct WideCfg {
     static constexpr int kLoads = 4 + TN / 64;               // ring loads per wavefront and chunk
     static constexpr int kBlockRows = (TN == 256) ? 4 : 8;   // tiles per XCD round: WG per CU of the XCD
     static constexpr int kBlockCols = kCusPerXcd * WG / kBlockRows;
-    static constexpr bool kPreload = (WG == 1);
 };
-using WideBig = WideCfg<256, 4, 1>;
 using WideTwo = WideCfg<128, 3, 2>;
 
 template <int TN, int STAGES, int WG>
@@ -1378,29 +1375,12 @@ __global__ void __launch_bounds__(256, WG) lu_gemm_f16w_kernel(float* A, int64_t
         const int r = r0 + lr;
         return *reinterpret_cast<const h8*>(&ring[st][o * kWB * kHK + r * kHK + wide_seg(r, lg) * 8]);
     };
-    // With one workgroup per CU, interior tiles read the first half of their
-    // C tile during the K loop: 32 loads issued in the first iteration after
-    // its chunk loads, consumed after the loop; they are younger than chunks
-    // 1..STAGES-1, so those iterations allow 32 more loads in flight.
     const bool interior = vec && rbase + kWB <= n && cbase + TN <= n;
-    constexpr int JP = C::kPreload ? JB / 2 : 1;
-    f4 cpre[JP][8];
     for (int ch = 0; ch < nchunks; ++ch) {
-        if (C::kPreload && interior && ch >= 1 && ch <= STAGES - 1)
-            __builtin_amdgcn_s_waitcnt(vm_wait((STAGES - 2) * C::kLoads + 32));
-        else
-            __builtin_amdgcn_s_waitcnt(vm_wait((STAGES - 2) * C::kLoads));  // this wavefront's chunk ch has landed
+        __builtin_amdgcn_s_waitcnt(vm_wait((STAGES - 2) * C::kLoads));  // this wavefront's chunk ch has landed
         stage_barrier();  // ... and every wavefront's; stage (ch - 1) % STAGES has no readers left
         const int st = ch % STAGES;
         issue(min(ch + STAGES - 1, nchunks - 1), (ch + STAGES - 1) % STAGES);
-        if (C::kPreload && ch == 0 && interior) {
-#pragma unroll
-            for (int jj = 0; jj < JP; ++jj)
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    cpre[jj][i] = *reinterpret_cast<const f4*>(A + (int64_t)(cbase + wc + 16 * jj + lr) * lda + rbase +
-                                                               wr + 16 * i + 4 * lg);
-        }
         h8 a[8], b[JB];
 #pragma unroll
         for (int i = 0; i < 8; ++i) a[i] = frag(st, 0, wr + 16 * i);
@@ -1416,9 +1396,8 @@ __global__ void __launch_bounds__(256, WG) lu_gemm_f16w_kernel(float* A, int64_t
         __builtin_amdgcn_sched_group_barrier(0x008, 8 * JB, 0);
     }
     __builtin_amdgcn_s_waitcnt(vm_wait(0));  // no load into LDS outlives the workgroup
-    // C += (-L)(U s) / s.  Interior tiles: no per-element conditions; a
-    // preloaded first half, then the rest with all its loads in flight at
-    // once (at most 32 per lane); edge tiles are guarded.
+    // C += (-L)(U s) / s.  Interior tiles: no per-element conditions, all of
+    // the wavefront's C loads in flight at once; edge tiles are guarded.
     float inv[JB];
 #pragma unroll
     for (int j = 0; j < JB; ++j) {
@@ -1426,33 +1405,22 @@ __global__ void __launch_bounds__(256, WG) lu_gemm_f16w_kernel(float* A, int64_t
         inv[j] = (col < n) ? 1.f / colscale[col] : 0.f;
     }
     if (interior) {
-        constexpr int H = C::kPreload ? 2 : 1, JH = JB / H;
+        f4 c[JB][8];
 #pragma unroll
-        for (int h = 0; h < H; ++h) {
-            __builtin_amdgcn_sched_barrier(0);
-            f4 c[JH][8];
+        for (int j = 0; j < JB; ++j)
 #pragma unroll
-            for (int jj = 0; jj < JH; ++jj)
+            for (int i = 0; i < 8; ++i)
+                c[j][i] = *reinterpret_cast<const f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda + rbase + wr +
+                                                       16 * i + 4 * lg);
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int j = JH * h + jj;
-                    if (C::kPreload && h == 0)
-                        c[jj][i] = cpre[jj % JP][i];
-                    else
-                        c[jj][i] = *reinterpret_cast<const f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda + rbase +
-                                                                wr + 16 * i + 4 * lg);
-                }
+        for (int j = 0; j < JB; ++j)
 #pragma unroll
-            for (int jj = 0; jj < JH; ++jj)
+            for (int i = 0; i < 8; ++i) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int j = JH * h + jj;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) c[jj][i][r] = fmaf(acc[i][j][r], inv[j], c[jj][i][r]);
-                    *reinterpret_cast<f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda + rbase + wr + 16 * i +
-                                           4 * lg) = c[jj][i];
-                }
-        }
+                for (int r = 0; r < 4; ++r) c[j][i][r] = fmaf(acc[i][j][r], inv[j], c[j][i][r]);
+                *reinterpret_cast<f4*>(A + (int64_t)(cbase + wc + 16 * j + lr) * lda + rbase + wr + 16 * i + 4 * lg) =
+                    c[j][i];
+            }
         return;
     }
 #pragma unroll

@@ -962,27 +962,31 @@ int rf_sim::solve_system(int maxit, double tol, int* iters_out, double* be_out)
 }
 
 // Rounding noise of the matrix-free FP64 operator at the current
-// configuration: ||A x - A x1 - A x2|| / bn with x = d_x split into two
-// pseudo-random summands (exact arithmetic gives 0).  Where two fibers come
+// configuration, from residual evaluations: with r(v) = b - A v and
+// x1 + x2 = 2 x, 2 r(x) - r(x1) - r(x2) = A x1 + A x2 - 2 A x is 0 in exact
+// arithmetic; x1 = 2 u x and x2 = 2 (1 - u) x with u pseudo-random per
+// entry.  d_res64 holds r(x) on entry and keeps it.  Where two fibers come
 // within ~1e-3 of each other's axis the analytic moments reach ~1e12 and the
-// FP64 evaluation of those rows carries rounding errors far above 1e-16 ||b||.
-// Uses d_fw, d_fZ[0], d_dx and d_res64 as scratch.
+// FP64 evaluation of those rows carries rounding errors far above
+// 1e-16 ||b||.  Between restart cycles the flexible GMRES buffers are free:
+// d_fV slots 0 and 1, d_fZ slot 0, d_dx and d_fw serve as scratch.
 int rf_sim::operator_noise(double bn, double* eta)
 {
     int rc;
-    double* x1 = d_fZ;
-    double* x2 = d_dx;
-    double* y = d_fw;
-    HIP_TRY(launch_random_split(n, d_x, x1, x2, stream));
-    if ((rc = apply_operator(kOpAx, d_x, y))) return rc;
-    if ((rc = apply_operator(kOpAx, x1, d_res64))) return rc;
-    HIP_TRY(launch_axpby(n, 1.0, y, -1.0, d_res64, y, stream));
-    if ((rc = apply_operator(kOpAx, x2, d_res64))) return rc;
-    HIP_TRY(launch_axpby(n, 1.0, y, -1.0, d_res64, y, stream));
-    double e2;
-    if ((rc = norm2(y, &e2))) return rc;
+    double* v0 = d_fV;
+    double* v1 = d_fV + lda;
+    double* z0 = d_fZ;
+    HIP_TRY(launch_random_split(n, d_x, v0, v1, stream));  // v0 + v1 = x
+    HIP_TRY(launch_axpby(n, 2.0, v0, 0.0, v0, z0, stream));  // x1
+    HIP_TRY(launch_axpby(n, 2.0, v1, 0.0, v1, v0, stream));  // x2
+    if ((rc = apply_operator(kOpResidual, z0, d_fw))) return rc;
+    if ((rc = apply_operator(kOpResidual, v0, d_dx))) return rc;
+    HIP_TRY(launch_axpby(n, 2.0, d_res64, -1.0, d_fw, v1, stream));
+    HIP_TRY(launch_axpby(n, 1.0, v1, -1.0, d_dx, z0, stream));
+    double e2, rr;
+    if ((rc = norm2(z0, &e2))) return rc;
     *eta = std::sqrt(e2) / bn;
-    return RF_OK;
+    return norm2(d_res64, &rr);  // ||r||^2 back in d_dots[0] for the next cycle
 }
 
 // Mixed precision: flexible GMRES on the FP64 system, with the FP32 matrix's
@@ -1039,9 +1043,14 @@ int rf_sim::solve_refined(int* iters_out, double* be_out)
     int total = 0, outer = 0;
     double be = 0.0;
     // the FP64 residual b - A x; ||r||^2 stays in d_dots[0] for the scaling
+    // the residual's source vectors equal the velocity's unless the analytic
+    // rhs drops an x/y force component (fortran/matrix_assembly.f90:857-859)
+    const bool fusable = !p.use_analytical_integration ||
+                         (p.external_force[0] == 0.0 && p.external_force[1] == 0.0);
     auto residual = [&](double* beta) -> int {
         int r = apply_operator(kOpResidual, d_x, d_res64);
         if (r) return r;
+        op_velocity_ready = fusable;  // until the next evaluation
         double rr;
         if ((r = norm2(d_res64, &rr))) return r;
         *beta = std::sqrt(rr);
@@ -1121,7 +1130,6 @@ int rf_sim::solve_refined(int* iters_out, double* be_out)
                 double eta = 0.0;
                 if ((rc = operator_noise(bn, &eta))) return rc;
                 stats.last_operator_noise = eta;
-                if ((rc = residual(&beta))) return rc;  // d_res64 and d_dots[0] for the next cycle
             }
             if (be <= kNoiseFactor * stats.last_operator_noise) {
                 stats.last_noise_limited = 1;
@@ -1132,11 +1140,6 @@ int rf_sim::solve_refined(int* iters_out, double* be_out)
     stats.last_outer_iters = outer;
     *iters_out = total;
     *be_out = be;
-    // the last evaluation was the residual at the final x; its sources equal
-    // the velocity's unless the analytic rhs drops an x/y force component
-    // (fortran/matrix_assembly.f90:857-859)
-    op_velocity_ready = !p.use_analytical_integration ||
-                        (p.external_force[0] == 0.0 && p.external_force[1] == 0.0);
     return RF_OK;
 }
 

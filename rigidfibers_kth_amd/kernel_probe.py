@@ -39,6 +39,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("kernel", choices=["gemv", "copy", "lu", "assemble", "steps"])
     ap.add_argument("--warmup", type=int, default=20, help="steps: untimed steps first")
+    ap.add_argument("--switch-iters", type=int, default=0,
+                    help="steps: neighbour-preconditioned iterations before the fp16 LU (0 = library default)")
     ap.add_argument("--fibers", type=int, default=1000)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--bytes", type=int, default=1 << 30)
@@ -55,7 +57,7 @@ def main(argv=None):
         return 0
     if a.kernel == "steps":
         kw = dict(precision=PREC_MIXED, profile=1, gmres_restart=30, gmres_max_iters=3000, gmres_tol=1e-10,
-                  precond=PRECOND_AUTO, lu_update_fp16=1)
+                  precond=PRECOND_AUTO, lu_update_fp16=1, precond_switch_iters=a.switch_iters)
         with _sim(a.fibers, **kw) as sim:
             for _ in range(a.warmup):
                 sim.step()

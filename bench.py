@@ -478,7 +478,7 @@ def main():
                          "per-step shadow error over all K steps (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the multi-GPU row sharding is a design only "
+        print("bench.py measures one GPU; the multi-GPU row sharding is a design only "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import rigidfibers_kth_amd as rf

@@ -67,7 +67,7 @@ extern "C" {
 #define RF_SOLVER_LU_FALLBACK 2
 
 /* Run parameters.  Lines 0..13 of the v1 parameter file, in order, then the
- * extensions this build adds (all default to the reference behaviour). */
+ * extensions this library adds (all default to the reference behaviour). */
 typedef struct rf_params {
     int32_t run_label;                    /* line 0 */
     int32_t layout_label;                 /* line 1 */

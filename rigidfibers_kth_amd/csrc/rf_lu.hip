@@ -440,75 +440,98 @@ __device__ __forceinline__ double wave_max(double v)
     return lane_value(v, 63);
 }
 
-// Diagonal block of width nb <= 64 by one wavefront: lane r holds row r (64
-// values in registers, the column loop fully unrolled), so a column step
-// needs no barrier and no LDS.  Rows are not moved while the block is
-// factorised: each lane keeps the position its row holds in LAPACK's
-// swapped order (the pivot search breaks ties by that position, as idamax
-// does), the pivot row's entries are broadcast with readlane, and every row
-// is stored at its final position at the end.  bak receives the original
-// values, ipiv and info[1] as in lu_diag_kernel.
+// Diagonal block of width nb <= 64 by four wavefronts: lane r of wavefront w
+// holds row r of the columns [16 w, 16 w + 16) in registers (the column loop
+// fully unrolled).  For column j the wavefront that owns it finds the pivot
+// (the first largest |a_rj| over the rows not yet pivots, ties to the smaller
+// position in LAPACK's swapped order, as idamax), scales the column and
+// publishes the multipliers (0 for pivot rows) and the pivot lane in LDS;
+// after one barrier every wavefront subtracts l_r u_c from its columns c > j,
+// the pivot row's u_c coming from readlane.  Rows are not moved while the
+// block is factorised: each lane keeps the position its row holds in the
+// swapped order and every row is stored at its final position at the end.
+// Non-pivot rows take exactly the operations of a row-by-row elimination;
+// pivot rows take l = 0, which leaves them unchanged up to the sign of an
+// exact zero.  bak receives the original values, ipiv and info[1] as in
+// lu_diag_kernel.
+constexpr int kDiagCols = 16;  // columns per wavefront
 template <typename T>
-__global__ void __launch_bounds__(64) lu_diag64_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv, int* info,
-                                                       T* __restrict__ bak, int64_t ldb)
+__global__ void __launch_bounds__(256) lu_diag64_kernel(T* A, int64_t lda, int k0, int nb, int* ipiv, int* info,
+                                                        T* __restrict__ bak, int64_t ldb)
 {
-    const int r = threadIdx.x;
+    __shared__ T sl[2][64];
+    __shared__ int sp[2];
+    const int r = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;  // wavefront-uniform
+    const int c0 = w * kDiagCols;
     const bool live = r < nb;
-    T a[64];
+    T a[kDiagCols];
 #pragma unroll
-    for (int c = 0; c < 64; ++c) a[c] = (live && c < nb) ? A[(int64_t)(k0 + c) * lda + k0 + r] : T(0);
+    for (int c = 0; c < kDiagCols; ++c)
+        a[c] = (live && c0 + c < nb) ? A[(int64_t)(k0 + c0 + c) * lda + k0 + r] : T(0);
 #pragma unroll
-    for (int c = 0; c < 64; ++c)
-        if (live && c < nb) bak[(int64_t)c * ldb + r] = a[c];
+    for (int c = 0; c < kDiagCols; ++c)
+        if (live && c0 + c < nb) bak[(int64_t)(c0 + c) * ldb + r] = a[c];
     int pos = r, my_ipiv = r;
     bool done = !live;
     bool zero = false;
-    // compile-time column indices keep a[] in registers (a 64-step loop is too
-    // large for the unroller)
+    // compile-time column indices keep a[] in registers
     static_for<64>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        if (j < nb) {  // uniform
-            // first largest |a_rj| over the rows not yet pivots, ties to the smaller position
-            const double v = done ? -1.0 : fabs((double)a[j]);
-            const double vmax = wave_max(v);
-            const unsigned long long cand = __ballot(v == vmax);
-            int p;
-            if (__popcll(cand) == 1) {
-                p = __ffsll((unsigned long long)cand) - 1;
-            } else {  // exact ties (or no candidate left): the smallest position
-                int k = (v == vmax) ? pos : 1 << 20;
+        constexpr int owner = j / kDiagCols, jl = j % kDiagCols, par = j & 1;
+        if (j < nb) {  // uniform over the workgroup
+            if (w == owner) {
+                const double v = done ? -1.0 : fabs((double)a[jl]);
+                const double vmax = wave_max(v);
+                const unsigned long long cand = __ballot(v == vmax);
+                int p;
+                if (__popcll(cand) == 1) {
+                    p = __ffsll((unsigned long long)cand) - 1;
+                } else {  // exact ties (or no candidate left): the smallest position
+                    int k = (v == vmax) ? pos : 1 << 20;
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) k = min(k, __shfl_xor(k, off, 64));
-                const unsigned long long at = __ballot(v == vmax && pos == k);
-                p = at ? __ffsll((unsigned long long)at) - 1 : 0;
+                    for (int off = 32; off > 0; off >>= 1) k = min(k, __shfl_xor(k, off, 64));
+                    const unsigned long long at = __ballot(v == vmax && pos == k);
+                    p = at ? __ffsll((unsigned long long)at) - 1 : 0;
+                }
+                p = __builtin_amdgcn_readfirstlane(p);
+                if (!(vmax > 0.0)) zero = true;
+                const T piv = lane_value(a[jl], p);
+                const T inv = (piv != T(0)) ? T(1) / piv : T(0);
+                const bool pivot_row = done || r == p;
+                const T l = a[jl] * inv;
+                if (!pivot_row) a[jl] = l;
+                sl[par][r] = pivot_row ? T(0) : l;
+                if (r == 0) sp[par] = p;
             }
-            p = __builtin_amdgcn_readfirstlane(p);
+            __syncthreads();
+            const int p = __builtin_amdgcn_readfirstlane(sp[par]);
+            const T lr = sl[par][r];
             const int ppos = __builtin_amdgcn_readlane(pos, p);
-            if (!(vmax > 0.0)) zero = true;
             if (r == j) my_ipiv = ppos;
             if (r != p && pos == j) pos = ppos;  // the row at position j moves to the pivot's position
-            const T piv = lane_value(a[j], p);
-            const T inv = (piv != T(0)) ? T(1) / piv : T(0);
             if (r == p) {
                 pos = j;
                 done = true;
             }
-            const T l = a[j] * inv;
-            if (!done) a[j] = l;
-            static_for<64>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                if constexpr (c > j) {
-                    const T u = lane_value(a[c], p);
-                    if (!done) a[c] -= l * u;
-                }
-            });
+            if (w == owner) {
+                static_for<kDiagCols>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    if constexpr (c > jl) a[c] -= lr * lane_value(a[c], p);
+                });
+            } else if (w > owner) {
+                static_for<kDiagCols>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    a[c] -= lr * lane_value(a[c], p);
+                });
+            }
         }
     });
     if (live) {
 #pragma unroll
-        for (int c = 0; c < 64; ++c)
-            if (c < nb) A[(int64_t)(k0 + c) * lda + k0 + pos] = a[c];
-        ipiv[k0 + r] = k0 + my_ipiv;
+        for (int c = 0; c < kDiagCols; ++c)
+            if (c0 + c < nb) A[(int64_t)(k0 + c0 + c) * lda + k0 + pos] = a[c];
+        if (w == 0) ipiv[k0 + r] = k0 + my_ipiv;
     }
     if (r == 0 && zero) info[1] = 1;  // zero pivot: left to the column-by-column panel
 }
@@ -762,9 +785,11 @@ __global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, Ge
         }
 }
 
-// Matrix-core version: workgroup tile 128 x BN (BN = 128 in FP32, 64 in
-// FP64), 4 wavefronts of 64 x BN/2 (blocks of 16x16), K staged 32 at a time
-// in LDS in the matrix's precision.
+// Matrix-core version: workgroup tile BM x BN (BM = 128, or 64 for the
+// narrow updates inside a super-panel; BN = 128 in FP32, 64 in FP64),
+// 4 wavefronts of BM/2 x BN/2 (blocks of 16x16), K staged 32 at a time in LDS
+// in the matrix's precision.  Every element of C takes the same products in
+// the same order whatever BM is.
 // For v_mfma_*_16x16x4 lane l supplies A[l%16][l/16] and B[l/16][l%16];
 // accumulator r of lane l is D[4(l/16) + r][l%16] for the f32 instruction
 // and D[l/16 + 4r][l%16] for the f64 one (rf_selftest_mfma_layout).  The
@@ -772,7 +797,7 @@ __global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, Ge
 // of one column (one 16-byte access); the f64 product is taken transposed,
 // D = (L21 U12)^T, so that for each r a wavefront touches 16 consecutive
 // rows of 4 columns.
-constexpr int kBM = 128, kKC = 32, kLdsPad = 4;
+constexpr int kBM = 128, kBMNarrow = 64, kKC = 32, kLdsPad = 4;
 // columns per workgroup tile: 128 in FP32; 64 in FP64 keeps the accumulators
 // and the prefetch registers of a wavefront within two waves per SIMD
 template <typename T>
@@ -791,11 +816,11 @@ __device__ __forceinline__ void mfma_16x16x4(float a, float b, f4& acc)
 // Bs[kk][c] = U12[kc + kk, cbase + c].  With VEC the global reads are 16-byte
 // vectors along the contiguous dimension (rows of L21, K of U12) and are
 // issued into registers one chunk ahead of the matrix-core work.
-template <typename T>
+template <typename T, int BM>
 struct GemmStage {
     static constexpr int VL = 16 / (int)sizeof(T);
     typedef T vec_t __attribute__((ext_vector_type(VL)));
-    static constexpr int NA = kKC * kBM / VL / 256;  // vectors per thread
+    static constexpr int NA = kKC * BM / VL / 256;  // vectors per thread
     static constexpr int BN = gemm_bn<T>();
     static constexpr int NB = kKC * BN / VL / 256;
     vec_t a[NA], b[NB];
@@ -806,7 +831,7 @@ struct GemmStage {
 #pragma unroll
         for (int t = 0; t < NA; ++t) {
             const int v = threadIdx.x + 256 * t;
-            const int kk = v / (kBM / VL), r0 = (v % (kBM / VL)) * VL;
+            const int kk = v / (BM / VL), r0 = (v % (BM / VL)) * VL;
             const int row = rbase + r0;
             const T* src = A + (int64_t)(kc + kk) * lda + row;
             // FP64: -L21, the accumulators start at C (see the kernel)
@@ -832,12 +857,12 @@ struct GemmStage {
             }
         }
     }
-    __device__ __forceinline__ void store(T (&As)[kKC][kBM + kLdsPad], T (&Bs)[kKC][BN + kLdsPad]) const
+    __device__ __forceinline__ void store(T (&As)[kKC][BM + kLdsPad], T (&Bs)[kKC][BN + kLdsPad]) const
     {
 #pragma unroll
         for (int t = 0; t < NA; ++t) {
             const int v = threadIdx.x + 256 * t;
-            const int kk = v / (kBM / VL), r0 = (v % (kBM / VL)) * VL;
+            const int kk = v / (BM / VL), r0 = (v % (BM / VL)) * VL;
             *reinterpret_cast<vec_t*>(&As[kk][r0]) = a[t];
         }
 #pragma unroll
@@ -850,25 +875,26 @@ struct GemmStage {
     }
 };
 
-template <typename T>
+template <typename T, int BM>
 __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, GemmRange g)
 {
     using V4 = typename std::conditional<sizeof(T) == 8, d4, f4>::type;
     constexpr bool kF64 = sizeof(T) == 8;
     constexpr int BN = gemm_bn<T>(), JB = BN / 32;  // JB 16-column blocks per wavefront
-    __shared__ __attribute__((aligned(16))) T As[kKC][kBM + kLdsPad];
+    constexpr int IB = BM / 32;                     // IB 16-row blocks per wavefront
+    __shared__ __attribute__((aligned(16))) T As[kKC][BM + kLdsPad];
     __shared__ __attribute__((aligned(16))) T Bs[kKC][BN + kLdsPad];
-    const int rbase = g.r0 + blockIdx.x * kBM, cbase = g.c0 + blockIdx.y * BN;
+    const int rbase = g.r0 + blockIdx.x * BM, cbase = g.c0 + blockIdx.y * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = (wave >> 1) * 64, wc = (wave & 1) * (BN / 2);
+    const int wr = (wave >> 1) * (BM / 2), wc = (wave & 1) * (BN / 2);
     const int lr = lane & 15, lk = lane >> 4;
     // FP64: the accumulators start as the C tile (loads issued ahead of the
     // K loop) and the MFMAs add (-L21) U12, hiding the tile's read latency
     // behind the first chunk.  FP32 keeps zero accumulators and subtracts at
     // the end: preloading C there costs a wavefront per SIMD of occupancy.
-    V4 acc[4][JB];
+    V4 acc[IB][JB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
         for (int j = 0; j < JB; ++j) {
             if constexpr (kF64) {
@@ -882,9 +908,9 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, Ge
                 acc[i][j] = V4{0, 0, 0, 0};
             }
         }
-    GemmStage<T> st;
+    GemmStage<T, BM> st;
     // 16-byte vectors need rows and K offsets that are multiples of the vector
-    const bool vec = (g.ka % GemmStage<T>::VL == 0) && (g.r0 % GemmStage<T>::VL == 0) && (lda % GemmStage<T>::VL == 0);
+    const bool vec = (g.ka % GemmStage<T, BM>::VL == 0) && (g.r0 % GemmStage<T, BM>::VL == 0) && (lda % GemmStage<T, BM>::VL == 0);
     st.load(A, lda, g.r1, g.c1, rbase, cbase, g.ka, min(kKC, g.kb - g.ka), vec);
     for (int kc = g.ka; kc < g.kb; kc += kKC) {
         __syncthreads();  // the previous chunk's reads of As/Bs are done
@@ -893,13 +919,13 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, Ge
         if (kc + kKC < g.kb) st.load(A, lda, g.r1, g.c1, rbase, cbase, kc + kKC, min(kKC, g.kb - kc - kKC), vec);
 #pragma unroll
         for (int ks = 0; ks < kKC; ks += 4) {
-            T a[4], b[JB];
+            T a[IB], b[JB];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[ks + lk][wr + 16 * i + lr];
+            for (int i = 0; i < IB; ++i) a[i] = As[ks + lk][wr + 16 * i + lr];
 #pragma unroll
             for (int j = 0; j < JB; ++j) b[j] = Bs[ks + lk][wc + 16 * j + lr];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < IB; ++i)
 #pragma unroll
                 for (int j = 0; j < JB; ++j) {
                     if constexpr (kF64)
@@ -910,7 +936,7 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, Ge
         }
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
         for (int j = 0; j < JB; ++j) {
             if constexpr (kF64) {
@@ -1468,8 +1494,8 @@ void block_panel(T* A, int64_t lda, int n, int k0, int k1, int* ipiv, int* info,
             return;
         }
     }
-    // panels of at most 64 columns (all FP64 ones): one wavefront
-    hipLaunchKernelGGL(lu_diag64_kernel<T>, dim3(1), dim3(64), 0, s, A, lda, k0, nb, ipiv, info, bak, (int64_t)n);
+    // panels of at most 64 columns (all FP64 ones): four wavefronts
+    hipLaunchKernelGGL(lu_diag64_kernel<T>, dim3(1), dim3(256), 0, s, A, lda, k0, nb, ipiv, info, bak, (int64_t)n);
     if (k1 < n)
         hipLaunchKernelGGL((lu_l21_kernel<T, 64>), grid, dim3(256), 0, s, A, lda, n, k0, nb, info, bak, (int64_t)n);
 }
@@ -1500,8 +1526,15 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
                                    0, s, A, lda, g, colscale);
         } else if (use_mfma) {
             const int bn = gemm_bn<T>();
-            hipLaunchKernelGGL(lu_gemm_mfma_kernel<T>, dim3((rows + kBM - 1) / kBM, (cols + bn - 1) / bn), dim3(256),
-                               0, s, A, lda, g);
+            // the narrow updates (a panel's columns or rows) take 64-row tiles:
+            // twice the workgroups for the columns, no empty half for the rows
+            if (rows <= kBMNarrow || cols <= bn)
+                hipLaunchKernelGGL((lu_gemm_mfma_kernel<T, kBMNarrow>),
+                                   dim3((rows + kBMNarrow - 1) / kBMNarrow, (cols + bn - 1) / bn), dim3(256), 0, s, A,
+                                   lda, g);
+            else
+                hipLaunchKernelGGL((lu_gemm_mfma_kernel<T, kBM>), dim3((rows + kBM - 1) / kBM, (cols + bn - 1) / bn),
+                                   dim3(256), 0, s, A, lda, g);
         } else {
             hipLaunchKernelGGL(lu_gemm_valu_kernel<T>, dim3((rows + kTile - 1) / kTile, (cols + kTile - 1) / kTile),
                                dim3(256), 0, s, A, lda, g);

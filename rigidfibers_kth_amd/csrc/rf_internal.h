@@ -91,9 +91,11 @@ GemvPlan plan_gemv(int rows, int cols, int64_t ld, int elem_bytes, int num_cus);
 size_t gemv_parts_elems(const GemvPlan& p);
 // mode: 0 -> y = A x ; 1 -> y = b - A x ; scale applied to A x first.
 // ev0/ev1 (optional) are recorded around the partial-product kernel only.
+// With x_nrm2 the product is A (x / sqrt(*x_nrm2)), the scaled vector formed
+// as launch_scale_copy forms it and stored to x_out when that is not null.
 hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double* parts, double* y,
                        const double* b, int mode, hipStream_t s, hipEvent_t ev0 = nullptr,
-                       hipEvent_t ev1 = nullptr);
+                       hipEvent_t ev1 = nullptr, const double* x_nrm2 = nullptr, double* x_out = nullptr);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 // bytes (a multiple of 4, at most 64 KiB) into coherent pinned host memory
 hipError_t launch_to_host(const void* src, void* host_dst, size_t bytes, hipStream_t s,
@@ -107,11 +109,23 @@ hipError_t launch_reduce_splits(const double* parts, int splits, int64_t stride,
 // parts must hold kMaxDotBlocks*(k+1) doubles (multidot_blocks(n) <= kMaxDotBlocks); counter is a device int (zero on entry, reset on exit).
 hipError_t launch_multidot(const double* V, int64_t ldv, int k, const double* w, int n, double* parts,
                            unsigned int* counter, double* out, hipStream_t s);
+// A copy of `words` 32-bit words from device memory into coherent pinned host
+// memory that ends with `seq` stored to *seq_word at system scope (the
+// sequence-word protocol of launch_to_host); dst == nullptr: no copy.
+struct HostCopy {
+    const unsigned* src = nullptr;
+    unsigned* dst = nullptr;
+    int words = 0;
+    unsigned* seq_word = nullptr;
+    unsigned seq = 0;
+};
 // w -= V[:, :k] h, then out[0..k) = V^T w (with_dots) and out[nd] = |w|^2
 // (nd = k with dots, else 0), bit-identical to the update followed by
-// launch_multidot; one kernel for k <= 32.
+// launch_multidot; one kernel for k <= 32.  With hc.dst the same kernel then
+// copies hc to the host (after out is written), as launch_to_host would.
 hipError_t launch_orth_dots(const double* V, int64_t ldv, int k, const double* h, double* w, int n, int with_dots,
-                            double* parts, unsigned int* counter, double* out, hipStream_t s);
+                            double* parts, unsigned int* counter, double* out, hipStream_t s,
+                            const HostCopy& hc = HostCopy());
 hipError_t launch_scale_copy(const double* w, const double* scale_num, int scale_idx, int n, double* v,
                              hipStream_t s);
 // xc = x + V[:, :k] y, y in host memory (passed as kernel arguments)

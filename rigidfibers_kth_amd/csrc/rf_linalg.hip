@@ -45,11 +45,20 @@ __device__ __forceinline__ void unpack(const V& v, double (&o)[N])
 // Workgroup: 64*RPL consecutive rows x one column split.  Wavefront w of the
 // workgroup takes columns c0+w, c0+w+4, ...; each lane loads 16 bytes (RPL
 // rows) per column, so one wavefront load is a 1 KiB contiguous segment.
+// With xs.nrm2 the vector is x / sqrt(*xs.nrm2), formed exactly as
+// scale_copy_kernel forms it, and the workgroups of the first row block
+// store it to xs.out (each column once): the next Arnoldi vector and its
+// product in one pass.
+struct XScale {
+    const double* nrm2;
+    double* out;
+};
+
 template <typename T>
 __global__ void __launch_bounds__(kGemvThreads) gemv_parts_kernel(int cols, int64_t ld, const T* __restrict__ A,
                                                                   const double* __restrict__ x,
                                                                   double* __restrict__ parts, int64_t pstride,
-                                                                  int cols_per_split)
+                                                                  int cols_per_split, XScale xs)
 {
     using V = typename Vec16<T>::type;
     constexpr int RPL = Vec16<T>::n;
@@ -63,6 +72,9 @@ __global__ void __launch_bounds__(kGemvThreads) gemv_parts_kernel(int cols, int6
     double acc[RPL];
 #pragma unroll
     for (int r = 0; r < RPL; ++r) acc[r] = 0.0;
+    const bool scaled = xs.nrm2 != nullptr;
+    const double inv = scaled ? 1.0 / sqrt(*xs.nrm2) : 1.0;
+    const bool store_x = scaled && xs.out != nullptr && blockIdx.x == 0 && lane == 0;
     const T* Ab = A + row0;
     int c = c0 + wave;
     for (; c + 4 * (kGemvUnroll - 1) < c1; c += 4 * kGemvUnroll) {
@@ -71,7 +83,11 @@ __global__ void __launch_bounds__(kGemvThreads) gemv_parts_kernel(int cols, int6
 #pragma unroll
         for (int u = 0; u < kGemvUnroll; ++u) {
             v[u] = __builtin_nontemporal_load(reinterpret_cast<const V*>(Ab + (int64_t)(c + 4 * u) * ld));
-            xv[u] = x[c + 4 * u];
+            xv[u] = scaled ? x[c + 4 * u] * inv : x[c + 4 * u];
+        }
+        if (store_x) {
+#pragma unroll
+            for (int u = 0; u < kGemvUnroll; ++u) xs.out[c + 4 * u] = xv[u];
         }
 #pragma unroll
         for (int u = 0; u < kGemvUnroll; ++u) {
@@ -83,7 +99,8 @@ __global__ void __launch_bounds__(kGemvThreads) gemv_parts_kernel(int cols, int6
     }
     for (; c < c1; c += 4) {
         const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(Ab + (int64_t)c * ld));
-        const double xv = x[c];
+        const double xv = scaled ? x[c] * inv : x[c];
+        if (store_x) xs.out[c] = xv;
         double e[RPL];
         unpack(v, e);
 #pragma unroll
@@ -202,12 +219,16 @@ __global__ void orth_update_kernel(const double* __restrict__ V, int64_t ldv, in
 // orth_update_kernel then multidot_kernel exactly (same grid-stride rows per
 // thread, same wave and block sums, same partial layout), so the results are
 // bit-identical to the two-kernel sequence with one launch less.  k <= MAXK.
+// With hc.dst the last workgroup then does what to_host_kernel does: copies
+// hc.words words from hc.src (out included) to host memory and stores the
+// sequence word after them at system scope.
 template <int MAXK>
 __global__ void __launch_bounds__(kDotThreads) orth_dots_kernel(const double* __restrict__ V, int64_t ldv, int k,
                                                                 const double* __restrict__ h,
                                                                 double* __restrict__ w, int n, int nd,
                                                                 double* __restrict__ parts,
-                                                                unsigned int* counter, double* __restrict__ out)
+                                                                unsigned int* counter, double* __restrict__ out,
+                                                                HostCopy hc)
 {
     __shared__ double hs[MAXK];
     __shared__ double sh[MAXK + 1][kDotThreads / 64];
@@ -263,6 +284,15 @@ __global__ void __launch_bounds__(kDotThreads) orth_dots_kernel(const double* __
             out[jj] = t;
         }
         if (threadIdx.x == 0) atomicExch(counter, 0u);
+        if (hc.dst) {
+            __threadfence();
+            __syncthreads();
+            for (int i = threadIdx.x; i < hc.words; i += kDotThreads)
+                hc.dst[i] = __hip_atomic_load(&hc.src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence_system();
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(hc.seq_word, hc.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -379,8 +409,9 @@ size_t gemv_parts_elems(const GemvPlan& p)
 }
 
 hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double* parts, double* y, const double* b,
-                       int mode, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+                       int mode, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, const double* x_nrm2, double* x_out)
 {
+    const XScale xs{x_nrm2, x_out};
     const int rows_per_block = 64 * (16 / p.elem_bytes);
     if (p.ld % rows_per_block != 0) return hipErrorInvalidValue;  // loads assume whole row blocks
     const int64_t pstride = (int64_t)p.row_blocks * rows_per_block;
@@ -392,10 +423,10 @@ hipError_t launch_gemv(const GemvPlan& p, const void* A, const double* x, double
     }
     if (p.elem_bytes == 8)
         hipLaunchKernelGGL(gemv_parts_kernel<double>, grid, dim3(kGemvThreads), 0, s, p.cols, p.ld,
-                           static_cast<const double*>(A), x, parts, pstride, cps);
+                           static_cast<const double*>(A), x, parts, pstride, cps, xs);
     else
         hipLaunchKernelGGL(gemv_parts_kernel<float>, grid, dim3(kGemvThreads), 0, s, p.cols, p.ld,
-                           static_cast<const float*>(A), x, parts, pstride, cps);
+                           static_cast<const float*>(A), x, parts, pstride, cps, xs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (ev1) {
@@ -428,22 +459,25 @@ hipError_t launch_multidot(const double* V, int64_t ldv, int k, const double* w,
 }
 
 hipError_t launch_orth_dots(const double* V, int64_t ldv, int k, const double* h, double* w, int n, int with_dots,
-                            double* parts, unsigned int* counter, double* out, hipStream_t s)
+                            double* parts, unsigned int* counter, double* out, hipStream_t s, const HostCopy& hc)
 {
     const int nd = with_dots ? k : 0;
     const dim3 grid(multidot_blocks(n)), block(kDotThreads);
+    if (hc.dst && (hc.words < 0 || hc.words > 16384 || !hc.src || !hc.seq_word)) return hipErrorInvalidValue;
     if (k <= 16) {
-        hipLaunchKernelGGL(orth_dots_kernel<16>, grid, block, 0, s, V, ldv, k, h, w, n, nd, parts, counter, out);
+        hipLaunchKernelGGL(orth_dots_kernel<16>, grid, block, 0, s, V, ldv, k, h, w, n, nd, parts, counter, out, hc);
         return hipGetLastError();
     }
     if (k <= 32) {
-        hipLaunchKernelGGL(orth_dots_kernel<32>, grid, block, 0, s, V, ldv, k, h, w, n, nd, parts, counter, out);
+        hipLaunchKernelGGL(orth_dots_kernel<32>, grid, block, 0, s, V, ldv, k, h, w, n, nd, parts, counter, out, hc);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(orth_update_kernel, dim3((n + 255) / 256), dim3(256), 0, s, V, ldv, k, h, w, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_multidot(with_dots ? V : nullptr, ldv, nd, w, n, parts, counter, out, s);
+    e = launch_multidot(with_dots ? V : nullptr, ldv, nd, w, n, parts, counter, out, s);
+    if (e != hipSuccess || !hc.dst) return e;
+    return launch_to_host(hc.src, hc.dst, sizeof(unsigned) * (size_t)hc.words, s, hc.seq_word, hc.seq);
 }
 
 hipError_t launch_scale_copy(const double* w, const double* nrm2, int idx, int n, double* v, hipStream_t s)

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -264,27 +265,37 @@ void rf_sim::resolve_timed()
     ev_pending.swap(running);
 }
 
-int rf_sim::gemv_A(const double* x, double* y, const double* b, int mode)
+int rf_sim::gemv_A(const double* x, double* y, const double* b, int mode, const double* x_nrm2, double* x_out)
 {
     hipEvent_t a = nullptr, e = nullptr;
     if (p.profile) {
         int rc = timed_pair(&a, &e);
         if (rc) return rc;
     }
-    HIP_TRY(launch_gemv(planA, d_A, x, d_gparts, y, b, mode, stream, a, e));
+    HIP_TRY(launch_gemv(planA, d_A, x, d_gparts, y, b, mode, stream, a, e, x_nrm2, x_out));
     if (p.profile) timed_push(a, e, kGemvA);
     return RF_OK;
 }
 
 // The copy into h_dots whose sequence number is host_seq has landed: its
 // sequence word is written after its data at system scope.  Normally it has
-// by the time the event or the stream reports completion; otherwise wait for
-// the stream and look again, and report a stale copy as an error.
-int rf_sim::await_host_copy()
+// by the time the stream reports completion; otherwise wait for the stream
+// and look again, and report a stale copy as an error.  poll: the stream
+// holds more work after the copy, so watch the sequence word itself (up to
+// a second) before waiting for the whole stream.
+int rf_sim::await_host_copy(bool poll)
 {
     const unsigned* w = host_seq_word();
     for (int spin = 0; spin < 1024; ++spin)
         if (__atomic_load_n(w, __ATOMIC_ACQUIRE) == host_seq) return RF_OK;
+    if (poll) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (;;) {
+            for (int spin = 0; spin < 4096; ++spin)
+                if (__atomic_load_n(w, __ATOMIC_ACQUIRE) == host_seq) return RF_OK;
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) break;
+        }
+    }
     HIP_TRY(hipStreamSynchronize(stream));
     if (__atomic_load_n(w, __ATOMIC_ACQUIRE) == host_seq) return RF_OK;
     rf::set_error("the dot block copied to host memory is stale (sequence word " +
@@ -429,20 +440,30 @@ int rf_sim::solve_gmres(int* iters_out, double* be_out, int* status_out, int max
             w_ready = false;
             HIP_TRY(launch_multidot(d_V, ldv, k, d_w, n, d_dparts, d_counter, D1, stream));
             HIP_TRY(launch_orth_dots(d_V, ldv, k, D1, d_w, n, 1, d_dparts, d_counter, D2, stream));
-            HIP_TRY(launch_orth_dots(d_V, ldv, k, D2, d_w, n, 0, d_dparts, d_counter, D3, stream));
-            if (k < m) HIP_TRY(launch_scale_copy(d_w, D3, 0, n, d_V + (size_t)k * ldv, stream));
-            // The Hessenberg column goes to the host; the next basis vector is
-            // already on the device, so its product is queued before the host
-            // waits (it is discarded if this iteration ends the solve).
-            HIP_TRY(launch_to_host(d_dots, h_dots, sizeof(double) * 3 * (m + 2), stream, host_seq_word(),
-                                   ++host_seq));
-            HIP_TRY(hipEventRecord(ev_k0, stream));
-            if (k < m && total_before(iterOut, m, k) < maxit) {
-                if ((rc = matvec(k))) return rc;
+            // The second pass ends by copying the Hessenberg column to the
+            // host.  The next basis vector is then on the device, so its
+            // product is queued before the host waits (it is discarded if
+            // this iteration ends the solve); without a preconditioner the
+            // product forms v_k = w / |w| itself and stores it.
+            HostCopy hc;
+            hc.src = reinterpret_cast<const unsigned*>(d_dots);
+            hc.dst = reinterpret_cast<unsigned*>(h_dots);
+            hc.words = (int)(sizeof(double) / sizeof(unsigned)) * 3 * (m + 2);
+            hc.seq_word = host_seq_word();
+            hc.seq = ++host_seq;
+            HIP_TRY(launch_orth_dots(d_V, ldv, k, D2, d_w, n, 0, d_dparts, d_counter, D3, stream, hc));
+            const bool next = k < m && total_before(iterOut, m, k) < maxit;
+            if (next && !prec_ready) {
+                if ((rc = gemv_A(d_w, d_w, nullptr, 0, D3, d_V + (size_t)k * ldv))) return rc;
                 w_ready = true;
+            } else if (k < m) {
+                HIP_TRY(launch_scale_copy(d_w, D3, 0, n, d_V + (size_t)k * ldv, stream));
+                if (next) {
+                    if ((rc = matvec(k))) return rc;
+                    w_ready = true;
+                }
             }
-            HIP_TRY(hipEventSynchronize(ev_k0));
-            if ((rc = await_host_copy())) return rc;
+            if ((rc = await_host_copy(true))) return rc;
             for (int j = 0; j < k; ++j) H[j + (size_t)jH * ldh] = h_dots[j] + h_dots[(m + 2) + j];
             const double dnormw = std::sqrt(h_dots[2 * (m + 2)]);
             H[k + (size_t)jH * ldh] = dnormw;
@@ -1330,11 +1351,8 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
         set_error("rf_create: hipStreamCreate failed");
         return RF_EHIP;
     }
-    for (hipEvent_t* ev : {&s->ev_p0, &s->ev_p1, &s->ev_k0, &s->ev_k1}) {
-        // ev_k0 marks the Hessenberg column's copy into h_dots once per GMRES
-        // iteration; a system-scope release there stalls the queue ~15 us
-        const unsigned flags = (ev == &s->ev_k0) ? (hipEventDisableTiming | hipEventDisableSystemFence) : 0u;
-        if (hipEventCreateWithFlags(ev, flags) != hipSuccess) {
+    for (hipEvent_t* ev : {&s->ev_p0, &s->ev_p1, &s->ev_k1}) {
+        if (hipEventCreateWithFlags(ev, 0u) != hipSuccess) {
             delete s;
             set_error("rf_create: hipEventCreate failed");
             return RF_EHIP;

@@ -21,7 +21,7 @@ struct rf_sim {
     rf::GemvPlan planA, planW;
     rf::OperatorPlan planOp;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_p0 = nullptr, ev_p1 = nullptr, ev_k0 = nullptr, ev_k1 = nullptr;
+    hipEvent_t ev_p0 = nullptr, ev_p1 = nullptr, ev_k1 = nullptr;
     std::vector<hipEvent_t> events;
     std::vector<void*> allocs;
 
@@ -80,7 +80,7 @@ struct rf_sim {
     double* h_dots = nullptr;  // pinned; its last double holds the copy's sequence word
     unsigned host_seq = 0;     // sequence number of the last copy into h_dots
     unsigned* host_seq_word() const { return reinterpret_cast<unsigned*>(h_dots + 3 * (restart + 2) + 7); }
-    int await_host_copy();
+    int await_host_copy(bool poll = false);
     double* h_y = nullptr;     // pinned
     // neighbourhoods and preconditioner (allocated on first use)
     int knn = 0;           // neighbours kept per fiber
@@ -139,7 +139,10 @@ struct rf_sim {
     int update_velocities();
     int advance();
     int step();
-    int gemv_A(const double* x, double* y, const double* b, int mode);
+    // y = A x (mode 0) or b - A x (mode 1); with x_nrm2, A (x / sqrt(*x_nrm2))
+    // and the scaled vector stored to x_out
+    int gemv_A(const double* x, double* y, const double* b, int mode, const double* x_nrm2 = nullptr,
+               double* x_out = nullptr);
     int fetch_dots(int count);
     int norm2(const double* v, double* out);
     int solve_gmres(int* iters, double* be, int* status, int maxit);

@@ -785,8 +785,9 @@ __global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, Ge
         }
 }
 
-// Matrix-core version: workgroup tile BM x BN (BM = 128, or 64 for the
-// narrow updates inside a super-panel; BN = 128 in FP32, 64 in FP64),
+// Matrix-core version: workgroup tile BM x BN (BM = 64 in FP64; in FP32 128,
+// or 64 for the narrow updates inside a super-panel; BN = 128 in FP32, 64 in
+// FP64),
 // 4 wavefronts of BM/2 x BN/2 (blocks of 16x16), K staged 32 at a time in LDS
 // in the matrix's precision.  Every element of C takes the same products in
 // the same order whatever BM is.
@@ -797,9 +798,13 @@ __global__ void __launch_bounds__(256) lu_gemm_valu_kernel(T* A, int64_t lda, Ge
 // of one column (one 16-byte access); the f64 product is taken transposed,
 // D = (L21 U12)^T, so that for each r a wavefront touches 16 consecutive
 // rows of 4 columns.
-constexpr int kBM = 128, kBMNarrow = 64, kKC = 32, kLdsPad = 4;
+constexpr int kBM = 128, kBMNarrow = 64, kKC = 32;
+// LDS rows padded by 16 bytes (keeps the 16-byte staging stores aligned)
+template <typename T>
+constexpr int lds_pad() { return 16 / (int)sizeof(T); }
 // columns per workgroup tile: 128 in FP32; 64 in FP64 keeps the accumulators
-// and the prefetch registers of a wavefront within two waves per SIMD
+// and the prefetch registers of a wavefront within three waves per SIMD (at
+// 64-row tiles)
 template <typename T>
 constexpr int gemm_bn() { return sizeof(T) == 8 ? 64 : 128; }
 
@@ -857,7 +862,7 @@ struct GemmStage {
             }
         }
     }
-    __device__ __forceinline__ void store(T (&As)[kKC][BM + kLdsPad], T (&Bs)[kKC][BN + kLdsPad]) const
+    __device__ __forceinline__ void store(T (&As)[kKC][BM + lds_pad<T>()], T (&Bs)[kKC][BN + lds_pad<T>()]) const
     {
 #pragma unroll
         for (int t = 0; t < NA; ++t) {
@@ -882,8 +887,8 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, Ge
     constexpr bool kF64 = sizeof(T) == 8;
     constexpr int BN = gemm_bn<T>(), JB = BN / 32;  // JB 16-column blocks per wavefront
     constexpr int IB = BM / 32;                     // IB 16-row blocks per wavefront
-    __shared__ __attribute__((aligned(16))) T As[kKC][BM + kLdsPad];
-    __shared__ __attribute__((aligned(16))) T Bs[kKC][BN + kLdsPad];
+    __shared__ __attribute__((aligned(16))) T As[kKC][BM + lds_pad<T>()];
+    __shared__ __attribute__((aligned(16))) T Bs[kKC][BN + lds_pad<T>()];
     const int rbase = g.r0 + blockIdx.x * BM, cbase = g.c0 + blockIdx.y * BN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave >> 1) * (BM / 2), wc = (wave & 1) * (BN / 2);
@@ -1526,9 +1531,11 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
                                    0, s, A, lda, g, colscale);
         } else if (use_mfma) {
             const int bn = gemm_bn<T>();
-            // the narrow updates (a panel's columns or rows) take 64-row tiles:
-            // twice the workgroups for the columns, no empty half for the rows
-            if (rows <= kBMNarrow || cols <= bn)
+            // FP64 takes 64-row tiles everywhere (three workgroups per CU
+            // instead of two: 85.7 against 91 ms for n = 15000); FP32 the
+            // narrow updates (a panel's columns or rows): twice the
+            // workgroups for the columns, no empty half for the rows
+            if (sizeof(T) == 8 || rows <= kBMNarrow || cols <= bn)
                 hipLaunchKernelGGL((lu_gemm_mfma_kernel<T, kBMNarrow>),
                                    dim3((rows + kBMNarrow - 1) / kBMNarrow, (cols + bn - 1) / bn), dim3(256), 0, s, A,
                                    lda, g);

@@ -121,6 +121,7 @@ def _stats_delta(s0, s1, steps):
         "velocity_gemv_avg_ms": d("velocity_gemv_ms_total") / max(1, d("velocity_gemv_launches")),
         "lu_factorizations": d("lu_factorizations"),
         "lu_avg_ms": d("lu_ms_total") / max(1, d("lu_factorizations")),
+        "lu_panels_redone": d("lu_panel_fallbacks"),
         "operator_launches_per_step": d("operator_launches") / steps,
         "operator_avg_ms": d("operator_ms_total") / max(1, d("operator_launches")),
     }
@@ -396,6 +397,7 @@ def long_run(rf, name, steps):
         st = sim.stats()
         sim.close()
         runs[mname] = {"snaps": snaps, "steps_per_sec": steps / wall, "lu_factorizations": int(st.lu_factorizations),
+                       "lu_panels_redone": int(st.lu_panel_fallbacks),
                        "gmres_iters": int(st.total_gmres_iters), "max_backward_error": worst_be,
                        "noise_limited_steps": noise_limited}
     names = [m for m, _ in spec["methods"]]

@@ -21,17 +21,12 @@ namespace rf {
 // |l| <= 1, no zero pivot); a panel that fails is restored and factorised
 // column by column (block_pivot = 0: every panel that way), so the pivots
 // are partial pivoting's either way.  *redone (may be null) receives the
-// number of panels redone.  With src (the matrix A holds a copy of: same
-// precision, or FP64 for FP32 factors) the checks are read once, after the
-// whole block-pivoted pass, and only if a panel failed is A copied again
-// from src and factorised with the check read per panel; without src the
-// stream is synchronised once per panel.  The checks are read through
-// host_flag (one int of coherent pinned host memory) when it is not null.
-// The factors are the same either way.  info[0..1] are zero on entry.
+// number of panels redone.  Block pivoting synchronises the stream once per
+// panel, reading the check through host_flag (one int of coherent pinned
+// host memory) when it is not null.  info[0..1] are zero on entry.
 size_t lu_workspace_bytes(int n, int update_fp16);
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     int block_pivot, int update_fp16, void* ws, int* redone, int* host_flag, const void* src,
-                     int src_elem_bytes, hipStream_t s);
+                     int block_pivot, int update_fp16, void* ws, int* redone, int* host_flag, hipStream_t s);
 // x <- U^-1 L^-1 P x with FP64 vectors, one launch per 64-row block and
 // sweep (FP32 or FP64 factors).  aux holds
 // lu_solve_aux_bytes(n, elem_bytes): the row permutation and the inverses of

@@ -4,10 +4,9 @@
 // Blocked right-looking factorisation, column-major, panel width NB, the
 // trailing update delayed to every second panel (super-panels of 2 NB):
 //   for each panel [k0, k1) (first bringing its columns up to date):
-//     either the block-pivoted panel (diagonal block in registers of four
-//       wavefronts, L21 = A21 U11^-1, checked against partial pivoting) or
-//       the column-by-column (Crout) panel with the pivot search over the
-//       whole column;
+//     either the block-pivoted panel (diagonal block in LDS, L21 = A21 U11^-1,
+//       checked against partial pivoting) or the column-by-column (Crout)
+//       panel with the pivot search over the whole column;
 //     apply the panel's row swaps to the other columns (laswp);
 //     U12 = L11^-1 A12 (unit lower triangular solve, the panel's rows first
 //       brought up to date);
@@ -259,8 +258,8 @@ __device__ __forceinline__ void load_block(const T* A, int64_t lda, int k0, int 
 }
 
 // Diagonal block [k0, k0+nb) factorised with pivot search inside the block;
-// ipiv[k0..k0+nb) as LAPACK (0-based global rows); bak (when not null)
-// receives the block's original values.  One workgroup of kDiagThreads: thread t owns column
+// ipiv[k0..k0+nb) as LAPACK (0-based global rows); bak receives the block's
+// original values.  One workgroup of kDiagThreads: thread t owns column
 // t % NB and the rows r = t / NB (mod kDiagThreads / NB).  Two barriers per column: (A) the pivot
 // row swap and the scaling of column j, (B) the rank-1 update, in which the
 // owners of column j+1 also find their partial argmax for the next column.
@@ -286,7 +285,7 @@ __global__ void __launch_bounds__(kDiagThreads) lu_diag_kernel(T* A, int64_t lda
     if (t == 0) zero_l = 0;
     load_block<T, NB, kDiagThreads>(A, lda, k0, nb, [&](int r, int cc, T v) {
         S[r][cc] = v;
-        if (bak) bak[(int64_t)cc * ldb + r] = v;
+        bak[(int64_t)cc * ldb + r] = v;
     });
     __syncthreads();
     if (c == 0) {  // partial argmax of column 0
@@ -472,7 +471,7 @@ __global__ void __launch_bounds__(256) lu_diag64_kernel(T* A, int64_t lda, int k
         a[c] = (live && c0 + c < nb) ? A[(int64_t)(k0 + c0 + c) * lda + k0 + r] : T(0);
 #pragma unroll
     for (int c = 0; c < kDiagCols; ++c)
-        if (bak && live && c0 + c < nb) bak[(int64_t)(c0 + c) * ldb + r] = a[c];
+        if (live && c0 + c < nb) bak[(int64_t)(c0 + c) * ldb + r] = a[c];
     int pos = r, my_ipiv = r;
     bool done = !live;
     bool zero = false;
@@ -599,7 +598,7 @@ __global__ void __launch_bounds__(256) lu_l21_kernel(T* A, int64_t lda, int n, i
     for (int k = 0; k < CW; ++k) {
         const int col = g * CW + k;
         a[k] = (live && col < nb) ? A[(int64_t)(k0 + col) * lda + row] : T(0);
-        if (bak && live && col < nb) bak[(int64_t)col * ldb + (row - k0)] = a[k];
+        if (live && col < nb) bak[(int64_t)col * ldb + (row - k0)] = a[k];
     }
 #pragma unroll
     for (int gg = 0; gg < 4; ++gg) {
@@ -1483,9 +1482,8 @@ hipError_t launch_wide(float* A, int64_t lda, int n, int K1, int kwp, const _Flo
     return hipGetLastError();
 }
 
-// Block-pivoted panel [k0, k1): diagonal block, then L21 below it; bak (when
-// not null) receives the panel's original columns (rows [k0, n), leading
-// dimension n).
+// Block-pivoted panel [k0, k1): diagonal block, then L21 below it; bak
+// receives the panel's original columns (rows [k0, n), leading dimension n).
 template <typename T>
 void block_panel(T* A, int64_t lda, int n, int k0, int k1, int* ipiv, int* info, T* bak, hipStream_t s)
 {
@@ -1511,22 +1509,10 @@ constexpr size_t kWsScaleOffset = kColMaxBlocks * (sizeof(double) + sizeof(int))
 size_t ws_backup_offset(int n) { return kWsScaleOffset + (sizeof(float) * (size_t)n + 255) / 256 * 256; }
 size_t ws_pack_offset(int n) { return ws_backup_offset(n) + (size_t)n * kMaxPanel * sizeof(float); }
 
-// Block pivoting runs in one of two ways.  With src (the matrix A was
-// copied from), no panel is checked on its own: the whole pass is checked
-// once at the end, and only if a panel failed is A copied again from src
-// and factorised the second way.  Without src, each panel's check is read
-// on the host before anything else uses the panel, and a failed panel is
-// restored from its copy and redone column by column.  Both give the second
-// way's factors: the first pass is kept only when no panel failed, and then
-// the second way would have redone nothing.
 template <typename T>
 hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, int block_pivot,
-                       int update_fp16, void* ws, int* redone, int* host_flag, const void* src, int src_elem_bytes,
-                       hipStream_t s)
+                       int update_fp16, void* ws, int* redone, int* host_flag, hipStream_t s)
 {
-    const bool once = block_pivot && src != nullptr;  // one check for the whole pass
-    if (once && src_elem_bytes != (int)sizeof(T) && !(src_elem_bytes == 8 && sizeof(T) == 4))
-        return hipErrorInvalidValue;
     double* pval = static_cast<double*>(ws);
     int* pidx = reinterpret_cast<int*>(pval + kColMaxBlocks);
     unsigned int* counter = reinterpret_cast<unsigned int*>(pidx + kColMaxBlocks);
@@ -1572,8 +1558,8 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
             const int k1 = std::min(K1, k0 + nb);
             gemm({k0, n, k0, k1, K0, k0});  // the panel's columns
             bool columns = !block_pivot;
-            if (block_pivot) block_panel<T>(A, lda, n, k0, k1, ipiv, info, once ? nullptr : bak, s);
-            if (block_pivot && !once) {
+            if (block_pivot) {
+                block_panel<T>(A, lda, n, k0, k1, ipiv, info, bak, s);
                 // the check decides before anything else reads the panel
                 int flag = 0;
                 hipError_t e;
@@ -1649,26 +1635,7 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (!once) return hipSuccess;
-    int flag = 0;
-    hipError_t e;
-    if (host_flag) {
-        e = launch_to_host(info + 1, host_flag, sizeof(int), s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        flag = *reinterpret_cast<volatile int*>(host_flag);
-    } else {
-        e = hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-    }
-    if (e != hipSuccess || !flag) return e;
-    // a panel failed: start again from the source, checking panel by panel
-    if (src_elem_bytes == (int)sizeof(T))
-        e = hipMemcpyAsync(A, src, sizeof(T) * (size_t)lda * n, hipMemcpyDeviceToDevice, s);
-    else
-        e = launch_f64_to_f32(static_cast<const double*>(src), reinterpret_cast<float*>(A), (size_t)lda * n, s);
-    if (e == hipSuccess) e = hipMemsetAsync(info, 0, 2 * sizeof(int), s);
-    if (e != hipSuccess) return e;
-    return factor_impl<T>(A, lda, n, nb, ipiv, info, use_mfma, 1, update_fp16, ws, redone, host_flag, nullptr, 0, s);
+    return hipSuccess;
 }
 
 // One wavefront: operands encode (mode 0) A[i][0] = i+1, B[0][j] = 1 -> D = i+1;
@@ -1822,15 +1789,14 @@ size_t lu_workspace_bytes(int n, int update_fp16)
 }
 
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
-                     int block_pivot, int update_fp16, void* ws, int* redone, int* host_flag, const void* src,
-                     int src_elem_bytes, hipStream_t s)
+                     int block_pivot, int update_fp16, void* ws, int* redone, int* host_flag, hipStream_t s)
 {
     if (nb < 1 || nb > (elem_bytes == 4 ? kMaxPanel : 64) || !ws || lda % 4 != 0) return hipErrorInvalidValue;
     if (elem_bytes == 8)
         return factor_impl<double>(static_cast<double*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, 0, ws,
-                                   redone, host_flag, src, src_elem_bytes, s);
+                                   redone, host_flag, s);
     return factor_impl<float>(static_cast<float*>(A), lda, n, nb, ipiv, info, use_mfma, block_pivot, update_fp16, ws,
-                              redone, host_flag, src, src_elem_bytes, s);
+                              redone, host_flag, s);
 }
 
 static size_t perm_bytes(int n) { return ((sizeof(int) * (size_t)n + 255) / 256) * 256; }

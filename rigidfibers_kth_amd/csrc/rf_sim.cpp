@@ -552,11 +552,8 @@ int rf_sim::factorize(bool* done, bool precond_only)
         if ((rc0 = timed_pair(&ea, &eb))) return rc0;
         HIP_TRY(hipEventRecord(ea, stream));
     }
-    // Block-pivoted panels, checked against partial pivoting.  After a
-    // factorisation in which no panel failed the check, the next one reads
-    // the checks once at the end (a failure then starts again from d_A,
-    // checking per panel); after one with failed panels (near-contact
-    // states keep some for many steps) it checks per panel from the start.
+    // block-pivoted panels, each checked against partial pivoting and redone
+    // column by column when the check fails
     int info[2] = {0, 0};
     if (es == (mixed ? 4 : 8))
         HIP_TRY(hipMemcpyAsync(d_LU, d_A, (size_t)lda * n * es, hipMemcpyDeviceToDevice, stream));
@@ -566,10 +563,8 @@ int rf_sim::factorize(bool* done, bool precond_only)
     lu_have = false;
     int redone = 0;
     HIP_TRY(lu_factor(d_LU, es, lda, n, nb, d_ipiv, d_info, p.lu_valu_update ? 0 : 1, p.lu_column_panel ? 0 : 1,
-                      fp16 ? 1 : 0, d_lu_ws, &redone, reinterpret_cast<int*>(h_dots),
-                      lu_check_per_panel ? nullptr : d_A, mixed ? 4 : 8, stream));
+                      fp16 ? 1 : 0, d_lu_ws, &redone, reinterpret_cast<int*>(h_dots), stream));
     stats.lu_panel_fallbacks += redone;
-    lu_check_per_panel = redone > 0;
     HIP_TRY(hipMemcpyAsync(h_dots, d_info, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
     if (p.profile) {
         HIP_TRY(hipEventRecord(eb, stream));
@@ -1912,26 +1907,23 @@ extern "C" int rf_selftest_lu(int32_t n, int32_t elem_bytes, int32_t nb, int32_t
             else
                 reinterpret_cast<float*>(host.data())[c * lda + r] = (float)v;
         }
-    void *dA = nullptr, *dS = nullptr, *dP = nullptr, *dW = nullptr;
+    void *dA = nullptr, *dP = nullptr, *dW = nullptr;
     hipError_t e = hipMalloc(&dA, host.size());
-    if (e == hipSuccess) e = hipMalloc(&dS, host.size());  // the source a failed block-pivoted pass restarts from
     if (e == hipSuccess) e = hipMalloc(&dP, sizeof(int) * ((size_t)n + 2));
     if (e == hipSuccess) e = hipMalloc(&dW, lu_workspace_bytes(n, update_fp16));
     if (e == hipSuccess) e = hipMemcpy(dA, host.data(), host.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dS, host.data(), host.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(dP, 0, sizeof(int) * ((size_t)n + 2));
     if (e == hipSuccess) e = hipMemset(dW, 0, lu_workspace_bytes(n, update_fp16));
     int* d_ipiv = static_cast<int*>(dP);
     int redone = 0;
     if (e == hipSuccess)
         e = lu_factor(dA, elem_bytes, lda, n, nb, d_ipiv, d_ipiv + n, valu ? 0 : 1, column_panel ? 0 : 1,
-                      update_fp16, dW, &redone, nullptr, dS, elem_bytes, nullptr);
+                      update_fp16, dW, &redone, nullptr, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     std::vector<int> piv((size_t)n + 2);
     if (e == hipSuccess) e = hipMemcpy(piv.data(), dP, sizeof(int) * ((size_t)n + 2), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(host.data(), dA, host.size(), hipMemcpyDeviceToHost);
     if (dA) (void)hipFree(dA);
-    if (dS) (void)hipFree(dS);
     if (dP) (void)hipFree(dP);
     if (dW) (void)hipFree(dW);
     if (e != hipSuccess) {

@@ -116,7 +116,6 @@ struct rf_sim {
     int auto_lu_run = 0;       // RF_PRECOND_AUTO: consecutive solves that ended on the LU
     static constexpr int kAutoRetry = 8;
     bool lu_fp16 = false;      // ... with fp16-rounded trailing updates (preconditioner use only)
-    bool lu_check_per_panel = false;  // the last factorisation redid panels: check the next one per panel
 
     rf_stats stats{};
 

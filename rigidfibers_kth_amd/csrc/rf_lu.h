@@ -22,8 +22,9 @@ namespace rf {
 // column by column (block_pivot = 0: every panel that way), so the pivots
 // are partial pivoting's either way.  *redone (may be null) receives the
 // number of panels redone.  Block pivoting synchronises the stream once per
-// panel, reading the check through host_flag (one int of coherent pinned
-// host memory) when it is not null.  info[0..1] are zero on entry.
+// panel, reading the check through host_flag (two ints of coherent pinned
+// host memory: the flag and the copy's sequence word, which the host polls)
+// when it is not null.  info[0..1] are zero on entry.
 size_t lu_workspace_bytes(int n, int update_fp16);
 hipError_t lu_factor(void* A, int elem_bytes, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma,
                      int block_pivot, int update_fp16, void* ws, int* redone, int* host_flag, hipStream_t s);

@@ -20,6 +20,7 @@
 #include "rf_lu.h"
 
 #include <algorithm>
+#include <chrono>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -1509,6 +1510,21 @@ constexpr size_t kWsScaleOffset = kColMaxBlocks * (sizeof(double) + sizeof(int))
 size_t ws_backup_offset(int n) { return kWsScaleOffset + (sizeof(float) * (size_t)n + 255) / 256 * 256; }
 size_t ws_pack_offset(int n) { return ws_backup_offset(n) + (size_t)n * kMaxPanel * sizeof(float); }
 
+// Until *word == seq (a copy kernel stores it after its data at system
+// scope): spins up to a second, then waits for the stream and looks again.
+static hipError_t wait_host_word(const unsigned* word, unsigned seq, hipStream_t s)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        for (int spin = 0; spin < 4096; ++spin)
+            if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return hipSuccess;
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) break;
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    return (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) ? hipSuccess : hipErrorUnknown;
+}
+
 template <typename T>
 hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, int use_mfma, int block_pivot,
                        int update_fp16, void* ws, int* redone, int* host_flag, hipStream_t s)
@@ -1564,9 +1580,13 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
                 int flag = 0;
                 hipError_t e;
                 if (host_flag) {  // own copy kernel: the runtime's device-to-host copy stalls the queue
-                    e = launch_to_host(info + 1, host_flag, sizeof(int), s);
-                    if (e == hipSuccess) e = hipStreamSynchronize(s);
-                    flag = *reinterpret_cast<volatile int*>(host_flag);
+                    // it ends with a sequence word in host_flag[1]: spin on that
+                    // rather than sleep in a stream synchronisation
+                    unsigned* seq_word = reinterpret_cast<unsigned*>(host_flag + 1);
+                    const unsigned seq = __atomic_load_n(seq_word, __ATOMIC_ACQUIRE) + 1u;
+                    e = launch_to_host(info + 1, host_flag, sizeof(int), s, seq_word, seq);
+                    if (e == hipSuccess) e = wait_host_word(seq_word, seq, s);
+                    flag = __atomic_load_n(host_flag, __ATOMIC_ACQUIRE);
                 } else {
                     e = hipMemcpyAsync(&flag, info + 1, sizeof(int), hipMemcpyDeviceToHost, s);
                     if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -975,8 +975,10 @@ __global__ void __launch_bounds__(256) lu_gemm_mfma_kernel(T* A, int64_t lda, Ge
 // C[:, j] = (C[:, j] s_j + (-L21) (U12[:, j] s_j)) / s_j with the operands
 // rounded to fp16 and FP32 accumulation (v_mfma_f32_16x16x32_f16).  |L21| <= 1
 // under partial pivoting, and s_j (a power of two from the U12 solve) puts
-// column j of U12 into [2^11, 2^12), inside fp16's normal range.  Tile 128 x
-// 128, 4 wavefronts of 64 x 64 (16x16 blocks), K staged 32 at a time in LDS
+// column j of U12 into [2^11, 2^12), inside fp16's normal range.  Tile TM x
+// TN (128 x 128, or 64 x 64 for the narrow updates of a 64-column panel:
+// twice the workgroups and no empty half), 4 wavefronts of TM/2 x TN/2
+// (16x16 blocks), K staged 32 at a time in LDS
 // as fp16 (double-buffered, global loads one chunk ahead); the accumulators
 // start as the scaled C tile.  The operand layout follows the 16x16x4 f32
 // instruction: lane l supplies row l%16 of A and column l%16 of B at the
@@ -986,18 +988,22 @@ typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 constexpr int kHB = 128, kHK = 32, kHPad = 8;
 
+template <int TM, int TN>
 struct HalfStage {
-    f4 a[4];  // A: rows 4*(t%32) .. +3 at K columns 4*(t/32) .. +3
-    f4 b[2][2];  // B: column v/4, K 8*(v%4) .. +7 for v = t, t + 256
-    float bs[2];
+    static constexpr int RG = TM / 4;        // 4-row groups of A
+    static constexpr int AT = RG * kHK / 4;  // threads that load A (4 rows x 4 K each)
+    static constexpr int BH = TN * 4 / 256;  // B vectors per thread
+    f4 a[4];  // A: rows 4*(t%RG) .. +3 at K columns 4*(t/RG) .. +3 (t < AT)
+    f4 b[BH][2];  // B: column v/4, K 8*(v%4) .. +7 for v = t + 256 h
+    float bs[BH];
 
     __device__ __forceinline__ void load(const float* A, int64_t lda, int r1, int c1, int rbase, int cbase, int kc,
                                          int kcount, bool vec, const float* __restrict__ colscale)
     {
         const int t = threadIdx.x;
-        const int r0 = rbase + 4 * (t % 32), kk0 = 4 * (t / 32);
+        const int r0 = rbase + 4 * (t % RG), kk0 = 4 * (t / RG);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 4 && (AT == 256 || t < AT); ++q) {
             const int kk = kk0 + q;
             const float* src = A + (int64_t)(kc + kk) * lda + r0;
             if (vec && kk < kcount && r0 + 3 < r1) {
@@ -1008,7 +1014,7 @@ struct HalfStage {
             }
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < BH; ++h) {
             const int v = t + 256 * h;
             const int col = cbase + v / 4, kb = 8 * (v % 4);
             const float* src = A + (int64_t)col * lda + kc + kb;
@@ -1025,19 +1031,19 @@ struct HalfStage {
             }
         }
     }
-    __device__ __forceinline__ void store(_Float16 (&As)[kHB][kHK + kHPad], _Float16 (&Bs)[kHB][kHK + kHPad]) const
+    __device__ __forceinline__ void store(_Float16 (&As)[TM][kHK + kHPad], _Float16 (&Bs)[TN][kHK + kHPad]) const
     {
         const int t = threadIdx.x;
-        const int r0 = 4 * (t % 32), kk0 = 4 * (t / 32);
+        const int r0 = 4 * (t % RG), kk0 = 4 * (t / RG);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 4 && (AT == 256 || t < AT); ++u) {
             h4 v;
 #pragma unroll
             for (int q = 0; q < 4; ++q) v[q] = (_Float16)(-a[q][u]);
             *reinterpret_cast<h4*>(&As[r0 + u][kk0]) = v;
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < BH; ++h) {
             const int v = t + 256 * h;
             h8 w;
 #pragma unroll
@@ -1051,8 +1057,9 @@ struct HalfStage {
 
 // Staging from the FP32 matrix (conversion and scaling on the way), one K
 // chunk of [ka, kb) per call.
+template <int TM, int TN>
 struct HalfFromMatrix {
-    HalfStage st;
+    HalfStage<TM, TN> st;
     const float* A;
     int64_t lda;
     int r1, c1, rbase, cbase, ka, kb;
@@ -1063,34 +1070,35 @@ struct HalfFromMatrix {
         const int kc = ka + chunk * kHK;
         st.load(A, lda, r1, c1, rbase, cbase, kc, min(kHK, kb - kc), vec, colscale);
     }
-    __device__ __forceinline__ void store(_Float16 (&As)[kHB][kHK + kHPad], _Float16 (&Bs)[kHB][kHK + kHPad]) const
+    __device__ __forceinline__ void store(_Float16 (&As)[TM][kHK + kHPad], _Float16 (&Bs)[TN][kHK + kHPad]) const
     {
         st.store(As, Bs);
     }
 };
 
-// One 128 x 128 tile C[rbase.., cbase..] (rows < r1, columns < c1) of the
+// One TM x TN tile C[rbase.., cbase..] (rows < r1, columns < c1) of the
 // fp16 update over nchunks K chunks from the staging object.
-template <class Stage>
+template <int TM, int TN, class Stage>
 __device__ __forceinline__ void f16_update_tile(float* A, int64_t lda, int rbase, int cbase, int r1, int c1, bool vec,
                                                 int nchunks, Stage& st, const float* __restrict__ colscale,
-                                                _Float16 (&As)[2][kHB][kHK + kHPad],
-                                                _Float16 (&Bs)[2][kHB][kHK + kHPad])
+                                                _Float16 (&As)[2][TM][kHK + kHPad],
+                                                _Float16 (&Bs)[2][TN][kHK + kHPad])
 {
+    constexpr int IB = TM / 32, JB = TN / 32;  // 16 x 16 blocks per wavefront
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    const int wr = (wave >> 1) * (TM / 2), wc = (wave & 1) * (TN / 2);
     const int lr = lane & 15, lg = lane >> 4;
-    float sc[4];
+    float sc[JB];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < JB; ++j) {
         const int col = cbase + wc + 16 * j + lr;
         sc[j] = (col < c1) ? colscale[col] : 1.f;
     }
-    f4 acc[4][4];
+    f4 acc[IB][JB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < JB; ++j) {
             const int row0 = rbase + wr + 16 * i + 4 * lg;
             const int col = cbase + wc + 16 * j + lr;
             const float* p = A + (int64_t)col * lda + row0;
@@ -1103,30 +1111,30 @@ __device__ __forceinline__ void f16_update_tile(float* A, int64_t lda, int rbase
         }
     st.load(0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] *= sc[j];
+        for (int j = 0; j < JB; ++j) acc[i][j] *= sc[j];
     int buf = 0;
     st.store(As[0], Bs[0]);
     for (int ch = 0; ch < nchunks; ++ch) {
         __syncthreads();  // chunk ch is in As/Bs[buf]; the other buffer's readers are done
         if (ch + 1 < nchunks) st.load(ch + 1);
-        h8 a[4], b[4];
+        h8 a[IB], b[JB];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const h8*>(&As[buf][wr + 16 * i + lr][8 * lg]);
+        for (int i = 0; i < IB; ++i) a[i] = *reinterpret_cast<const h8*>(&As[buf][wr + 16 * i + lr][8 * lg]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const h8*>(&Bs[buf][wc + 16 * j + lr][8 * lg]);
+        for (int j = 0; j < JB; ++j) b[j] = *reinterpret_cast<const h8*>(&Bs[buf][wc + 16 * j + lr][8 * lg]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < IB; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < JB; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
         if (ch + 1 < nchunks) st.store(As[buf ^ 1], Bs[buf ^ 1]);
         buf ^= 1;
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < IB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < JB; ++j) {
             const int row0 = rbase + wr + 16 * i + 4 * lg;
             const int col = cbase + wc + 16 * j + lr;
             if (col >= c1) continue;
@@ -1142,24 +1150,25 @@ __device__ __forceinline__ void f16_update_tile(float* A, int64_t lda, int rbase
         }
 }
 
+template <int TM, int TN>
 __global__ void __launch_bounds__(256) lu_gemm_f16_kernel(float* A, int64_t lda, GemmRange g,
                                                          const float* __restrict__ colscale)
 {
-    __shared__ __attribute__((aligned(16))) _Float16 As[2][kHB][kHK + kHPad];
-    __shared__ __attribute__((aligned(16))) _Float16 Bs[2][kHB][kHK + kHPad];
-    HalfFromMatrix st;
+    __shared__ __attribute__((aligned(16))) _Float16 As[2][TM][kHK + kHPad];
+    __shared__ __attribute__((aligned(16))) _Float16 Bs[2][TN][kHK + kHPad];
+    HalfFromMatrix<TM, TN> st;
     st.A = A;
     st.lda = lda;
     st.r1 = g.r1;
     st.c1 = g.c1;
-    st.rbase = g.r0 + blockIdx.x * kHB;
-    st.cbase = g.c0 + blockIdx.y * kHB;
+    st.rbase = g.r0 + blockIdx.x * TM;
+    st.cbase = g.c0 + blockIdx.y * TN;
     st.ka = g.ka;
     st.kb = g.kb;
     st.vec = (g.ka % 4 == 0) && (g.r0 % 4 == 0) && (lda % 4 == 0);
     st.colscale = colscale;
-    f16_update_tile(A, lda, st.rbase, st.cbase, g.r1, g.c1, st.vec, (g.kb - g.ka + kHK - 1) / kHK, st, colscale, As,
-                    Bs);
+    f16_update_tile<TM, TN>(A, lda, st.rbase, st.cbase, g.r1, g.c1, st.vec, (g.kb - g.ka + kHK - 1) / kHK, st,
+                            colscale, As, Bs);
 }
 
 // ---- packed fp16 operands of a super-panel's trailing update ---------------
@@ -1278,7 +1287,7 @@ __global__ void __launch_bounds__(256, 3) lu_gemm_f16p_kernel(float* A, int64_t 
     st.rn = n - K1;
     st.cn = n - K1;
     const bool vec = (K1 % 4 == 0) && (lda % 4 == 0);
-    f16_update_tile(A, lda, K1 + st.ri, K1 + st.cj, n, n, vec, kwp / kHK, st, colscale, As, Bs);
+    f16_update_tile<kHB, kHB>(A, lda, K1 + st.ri, K1 + st.cj, n, n, vec, kwp / kHK, st, colscale, As, Bs);
 }
 
 // The same update with wide tiles: 256 rows, and each wavefront a 128-row
@@ -1542,9 +1551,16 @@ hipError_t factor_impl(T* A, int64_t lda, int n, int nb, int* ipiv, int* info, i
         if (g.r1 <= g.r0 || g.c1 <= g.c0 || g.kb <= g.ka) return;
         const int rows = g.r1 - g.r0, cols = g.c1 - g.c0;
         if (f16) {
-            if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL(lu_gemm_f16_kernel, dim3((rows + kHB - 1) / kHB, (cols + kHB - 1) / kHB), dim3(256),
-                                   0, s, A, lda, g, colscale);
+            if constexpr (sizeof(T) == 4) {
+                // the narrow updates of a panel of at most 64 columns or rows
+                // take 64 x 64 tiles (same products per element)
+                if (rows <= 64 || cols <= 64)
+                    hipLaunchKernelGGL((lu_gemm_f16_kernel<64, 64>), dim3((rows + 63) / 64, (cols + 63) / 64),
+                                       dim3(256), 0, s, A, lda, g, colscale);
+                else
+                    hipLaunchKernelGGL((lu_gemm_f16_kernel<kHB, kHB>), dim3((rows + kHB - 1) / kHB, (cols + kHB - 1) / kHB),
+                                       dim3(256), 0, s, A, lda, g, colscale);
+            }
         } else if (use_mfma) {
             const int bn = gemm_bn<T>();
             // FP64 takes 64-row tiles everywhere (three workgroups per CU

@@ -1351,7 +1351,7 @@ extern "C" int rf_create(const rf_params* params, int32_t num_fibers, const doub
         set_error("rf_create: hipStreamCreate failed");
         return RF_EHIP;
     }
-    for (hipEvent_t* ev : {&s->ev_p0, &s->ev_p1, &s->ev_k1}) {
+    for (hipEvent_t* ev : {&s->ev_p0, &s->ev_p1}) {
         if (hipEventCreateWithFlags(ev, 0u) != hipSuccess) {
             delete s;
             set_error("rf_create: hipEventCreate failed");

@@ -21,7 +21,7 @@ struct rf_sim {
     rf::GemvPlan planA, planW;
     rf::OperatorPlan planOp;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_p0 = nullptr, ev_p1 = nullptr, ev_k1 = nullptr;
+    hipEvent_t ev_p0 = nullptr, ev_p1 = nullptr;
     std::vector<hipEvent_t> events;
     std::vector<void*> allocs;
 
